@@ -1,0 +1,1587 @@
+// prc_device.hip -- gfx950 kernels of the positron-range-correction path and the extern "C"
+// layer declared in include/prc.h.
+//
+// The system matrix A of the reference (prc.cu:167-267) is fixed for a reconstruction: it
+// depends on density, geometry and isotope only. Instead of re-marching every ray in each
+// forward and adjoint pass (prc.cu:178-230), the normalisation pass marches every ray once
+// and stores each per-step weight in a "row-plane" layout:
+//
+//   for each X row (y, z) holding active voxels, x in [xa, xa + len):
+//     for each ray p (p = 0: the 125-tap near block, p = 1 + r: far ray r):
+//       cnt(row, p) steps k = kfirst(p) .. kfirst(p) + cnt - 1, each a plane of len weights [x].
+//
+// For a fixed ray and step, the DDA voxel offset (dx, dy, dz) is the same for every source
+// (prc.cu:118-129, 140-142 depend on the voxel spacing only), so
+//   adjoint: out_s = sum_{p,k} w[row(s)][p][k][x_s] * in[s + D(p,k)] / W_s   (stream the planes)
+//   forward: out_t = sum_{p,k} w[row(t - D)][p][k][x_t - dx] * f/W[t - D]    (gather the planes)
+// are both gathers with a wave-uniform base address plus the lane index: no atomics, a fixed
+// summation order, and A^T is exactly the transpose of A (same stored weights).
+//
+// All kernels take a table of volume descriptors and a list of work items (vol, row, x0, n)
+// so one launch serves any number of volumes (the batched path).
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "prc.h"
+
+namespace {
+
+constexpr int kBins = PRC_RANGE_BINS;    // prc.cu:61
+constexpr int kNearRays = PRC_NEAR_RAYS; // prc.cu:104
+constexpr int kNQ = 125;                 // 5x5x5 near block (prc.cu:103)
+constexpr int kNearMax = 8;              // a near ray visits at most 7 block voxels
+constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
+constexpr int kWave = 64;
+constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
+
+// ---------------------------------------------------------------------------------------
+// Isotope model (prc.cu:38-57): Paneque-Yunta et al., Phys. Med. Biol. 70 (2025) 165013.
+struct Branch {
+    float w, a, b, c, f;
+};
+struct Isotope {
+    const char* name;
+    int n;
+    Branch br[3];
+};
+const Isotope kIsotopes[8] = {
+    {"11C", 1, {{1.000f, 0.777f, 0.468f, 0.217f, 0.010f}}},
+    {"13N", 1, {{1.000f, 0.537f, 0.347f, 0.193f, 0.050f}}},
+    {"15O", 1, {{1.000f, 0.295f, 0.221f, 0.133f, 0.118f}}},
+    {"18F", 1, {{1.000f, 1.612f, 0.703f, 0.526f, 0.011f}}},
+    {"64Cu", 1, {{1.000f, 1.494f, 0.000f, 0.631f, 0.251f}}},
+    {"68Ga", 2, {{0.014f, 1.048f, 0.000f, 0.475f, 0.271f}, {0.986f, 0.289f, 0.137f, 0.159f, 0.322f}}},
+    {"82Rb", 2, {{0.136f, 0.191f, 0.084f, 0.114f, 0.408f}, {0.857f, 0.135f, 0.063f, 0.087f, 0.451f}}},
+    {"124I", 3, {{0.013f, 1.007f, 0.000f, 0.421f, 0.443f}, {0.515f, 0.412f, 0.000f, 0.210f, 0.458f},
+                 {0.471f, 0.241f, 0.000f, 0.150f, 0.369f}}},
+};
+
+// The smallest branch absorbs the FP32 rounding of the published weights (prc.cu:577-584).
+Isotope renormalised(int idx) {
+    Isotope m = kIsotopes[idx];
+    float wsum = 0.0f;
+    int smallest = 0;
+    for (int b = 0; b < m.n; ++b) {
+        wsum += m.br[b].w;
+        if (m.br[b].w < m.br[smallest].w) smallest = b;
+    }
+    m.br[smallest].w += 1.0f - wsum;
+    return m;
+}
+
+// S(r) = sum_b w_b exp(-[(a r)^(1+f) + (b r)^2.5 + (c r)^3.5]) (prc.cu:65-72), evaluated
+// without contraction, as the reference's host table build does.
+__host__ __device__ inline float survival(const Branch* br, int n, float r) {
+    float s = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        const float x = br[i].b * r, y = br[i].c * r;
+        const float p = powf(br[i].a * r, 1.0f + br[i].f);
+        const float q = x * x * sqrtf(x);
+        const float u = y * y * y * sqrtf(y);
+        s = s + br[i].w * expf(-((p + q) + u));
+    }
+    return s;
+}
+
+// prc.cu:83-91 (host, FP32).
+float range_cutoff_host(const Isotope& m, float fraction) {
+    float lo = 0.0f, hi = 1.0f;
+    while (1.0f - survival(m.br, m.n, hi) < fraction && hi < 256.0f) hi *= 2.0f;
+    for (int it = 0; it < 64; ++it) {
+        const float mid = 0.5f * (lo + hi);
+        if (1.0f - survival(m.br, m.n, mid) < fraction) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+// ---------------------------------------------------------------------------------------
+// Device data layout.
+struct RowInfo {        // one per (row, ray)
+    long long off;      // element offset of the row's planes for this ray in the weight buffer
+    int cnt;            // stored steps
+    int kt;             // forward bound: last step (exclusive) at which this row receives weight
+};
+struct FarStep {        // march table of a far ray
+    float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
+    int dp;             // padded linear offset of the step's voxel from the source
+};
+struct NearStep {
+    float seg;
+    int dp;
+    int q;              // near-block bin 0..124
+    int loc;            // index of the bin inside its Manhattan shell (step s visits shell s)
+};
+
+// Near-block bins grouped by Manhattan distance from the source voxel. A DDA step moves one
+// voxel along one axis away from the source, so step s of a near ray always lands in shell s
+// (sizes 1, 6, 18, 32, 36, 24, 8): per-step accumulators stay small enough for register
+// indexing.
+struct ShellMap {
+    int shell[kNQ];
+    int local[kNQ];
+};
+constexpr int iabs_c(int v) { return v < 0 ? -v : v; }
+constexpr ShellMap make_shell_map() {
+    ShellMap m{};
+    int count[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < kNQ; ++q) {
+        const int sh = iabs_c(q % 5 - 2) + iabs_c(q / 5 % 5 - 2) + iabs_c(q / 25 - 2);
+        m.shell[q] = sh;
+        m.local[q] = count[sh]++;
+    }
+    return m;
+}
+constexpr ShellMap kShellMap = make_shell_map();
+
+struct Vol {            // device descriptor of one volume
+    int nx, ny, nz, n_rays;
+    float sx, sy, sz, cutoff, support_rho, air_cap, scale;
+    int nb, exact, L, LP;
+    Branch br[3];
+    size_t vbase;       // offset of this volume in concatenated voxel vectors
+    const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, +inf halo
+    const float* table; // kBins + 1
+    const unsigned char* support;
+    float* W;           // per voxel
+    const int* row_xa;  // [ny*nz]
+    const int* row_len; // [ny*nz], 0 = no active voxel
+    RowInfo* rinfo;     // [ny*nz][n_rays + 1]
+    void* wbuf;         // weights (float or __half)
+    const FarStep* fstep;   // [n_rays][L]
+    const int4* pstep;      // [n_rays + 1][LP] (dx, dy, dz) of each step; p = 0 is the near block
+    const int* kfirst;      // [n_rays + 1] first stored step
+    const NearStep* nstep;  // [kNearRays][kNearMax]
+    const int* nlen;        // [kNearRays]
+};
+
+// Solver scalars of one volume (device).
+struct Scal {
+    double support_sum;
+    double sum;          // last FP64 sum
+    double hh, hy;       // Biggs-Andrews dot products
+    float eps, lambda, scale, beta, gamma;
+    int accel;
+};
+
+// Loads and stores through the global address space. Pointers kept in descriptors are generic
+// pointers to the compiler; casting them lets wave-uniform reads become scalar loads and lets
+// per-lane accesses use global (not flat) instructions.
+#define GLOBAL_AS __attribute__((address_space(1)))
+typedef int v2i __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+template <class T>
+__device__ inline T gload(const T* p) {
+    if constexpr (sizeof(T) == 16) return __builtin_bit_cast(T, *(const GLOBAL_AS v4i*)p);
+    else if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, *(const GLOBAL_AS v2i*)p);
+    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, *(const GLOBAL_AS int*)p);
+    else if constexpr (sizeof(T) == 2) return __builtin_bit_cast(T, *(const GLOBAL_AS unsigned short*)p);
+    else return __builtin_bit_cast(T, *(const GLOBAL_AS unsigned char*)p);
+}
+
+template <class T>
+__device__ inline void gstore(T* p, T v) {
+    if constexpr (sizeof(T) == 4) *(GLOBAL_AS int*)p = __builtin_bit_cast(int, v);
+    else if constexpr (sizeof(T) == 2) *(GLOBAL_AS unsigned short*)p = __builtin_bit_cast(unsigned short, v);
+    else *(GLOBAL_AS T*)p = v;
+}
+
+template <class T> struct WeightIO;
+template <> struct WeightIO<float> {
+    __device__ static float load(const float* p) { return gload(p); }
+    __device__ static float store_value(float v) { return v; }
+    __device__ static float to_float(float v) { return v; }
+};
+template <> struct WeightIO<__half> {
+    __device__ static float load(const __half* p) { return __half2float(gload(p)); }
+    __device__ static __half store_value(float v) { return __float2half_rn(v); }
+    __device__ static float to_float(__half v) { return __half2float(v); }
+};
+
+__device__ inline float lookup(const float* tab, float scale, float w) {  // prc.cu:74-80
+    const float u = w * scale;
+    const int i = min((int)u, kBins - 1);
+    const float lo = tab[i], hi = tab[i + 1];
+    return fmaf(fminf(u - (float)i, 1.0f), hi - lo, lo);
+}
+
+// Index of the calling wave inside its block, known to the compiler as wave-uniform so that
+// item and descriptor loads become scalar loads.
+__device__ inline int wave_in_block() { return __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave)); }
+
+__device__ inline int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ inline long long wave_sum_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ inline double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Tables built on the device.
+
+__global__ void k_range_table(float* table, Branch b0, Branch b1, Branch b2, int nb, float scale) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j > kBins) return;
+    const Branch br[3] = {b0, b1, b2};
+    table[j] = survival(br, nb, (float)j / scale);  // prc.cu:625-633
+}
+
+// The ray set of prc.cu:118-129: mirrored pairs, Fibonacci hemisphere directions and R3 offsets.
+// sincospif is evaluated as the correctly rounded sin/cos of pi*arg via FP64.
+__device__ void ray_dir(int k, int n, float3* start, float3* dir) {
+    const unsigned m = (unsigned)k >> 1;
+    const float sign = (k & 1) ? -1.0f : 1.0f;
+    const float z = 1.0f - ((float)m + 0.5f) / (float)((n + 1) >> 1);
+    const float r = sqrtf(fmaxf(0.0f, fmaf(-z, z, 1.0f)));
+    const float to_unit = 2.3283064e-10f;  // 2^-32
+    const float arg = 2.0f * to_unit * (float)(m * 2654435769u);
+    const double pa = 3.14159265358979323846 * (double)arg;
+    const float sn = (float)sin(pa), cs = (float)cos(pa);
+    *dir = make_float3(sign * r * cs, sign * r * sn, sign * z);
+    *start = make_float3(sign * fmaf(to_unit, (float)(0x80000000u + m * 3518319155u), -0.5f),
+                         sign * fmaf(to_unit, (float)(0x80000000u + m * 2882110345u), -0.5f),
+                         sign * fmaf(to_unit, (float)(0x80000000u + m * 2360945575u), -0.5f));
+}
+
+// Geometric voxel walk of one ray (prc.cu:137-163 without the density): the voxel offset and
+// the in-voxel path length of every step are the same for all sources.
+struct Walk {
+    int di, dj, dk, i, j, k;
+    float tx, ty, tz, dtx, dty, dtz, t;
+    __device__ Walk(float3 x, float3 d, float sx, float sy, float sz) {
+        const float big = 1.0e30f;
+        di = d.x > 0.0f ? 1 : -1, dj = d.y > 0.0f ? 1 : -1, dk = d.z > 0.0f ? 1 : -1;
+        tx = d.x != 0.0f ? fmaf(0.5f, (float)di, -x.x) * sx / d.x : big;
+        ty = d.y != 0.0f ? fmaf(0.5f, (float)dj, -x.y) * sy / d.y : big;
+        tz = d.z != 0.0f ? fmaf(0.5f, (float)dk, -x.z) * sz / d.z : big;
+        dtx = d.x != 0.0f ? sx / fabsf(d.x) : big;
+        dty = d.y != 0.0f ? sy / fabsf(d.y) : big;
+        dtz = d.z != 0.0f ? sz / fabsf(d.z) : big;
+        i = j = k = 0;
+        t = 0.0f;
+    }
+    __device__ bool in_block() const { return abs(i) <= 2 && abs(j) <= 2 && abs(k) <= 2; }
+    __device__ float step() {  // returns the segment of the current voxel and advances
+        const float tn = fminf(tx, fminf(ty, tz));
+        const float seg = tn - t;
+        t = tn;
+        if (tn == tx) i += di, tx += dtx;
+        else if (tn == ty) j += dj, ty += dty;
+        else k += dk, tz += dtz;
+        return seg;
+    }
+};
+
+__global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* kfirst, NearStep* nstep, int* nlen) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxp = V.nx + 2, nyp = V.ny + 2;
+    if (id < V.n_rays) {
+        float3 x, d;
+        ray_dir(id, V.n_rays, &x, &d);
+        Walk w(x, d, V.sx, V.sy, V.sz);
+        int kf = -1;
+        for (int s = 0; s < V.L; ++s) {
+            if (kf < 0 && !w.in_block()) kf = s;
+            const int i = w.i, j = w.j, k = w.k;
+            pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
+            const float seg = w.step();
+            fstep[(size_t)id * V.L + s] = FarStep{seg, (k * nyp + j) * nxp + i};
+        }
+        kfirst[1 + id] = kf < 0 ? V.L : kf;
+    } else if (id < V.n_rays + kNearRays) {
+        const int r = id - V.n_rays;
+        float3 x, d;
+        ray_dir(r, kNearRays, &x, &d);
+        Walk w(x, d, V.sx, V.sy, V.sz);
+        int s = 0;
+        for (; s < kNearMax && w.in_block(); ++s) {
+            const int i = w.i, j = w.j, k = w.k;
+            const float seg = w.step();
+            const int q = ((k + 2) * 5 + j + 2) * 5 + i + 2, sh = abs(i) + abs(j) + abs(k);
+            int loc = 0;
+            for (int q2 = 0; q2 < q; ++q2) loc += abs(q2 % 5 - 2) + abs(q2 / 5 % 5 - 2) + abs(q2 / 25 - 2) == sh;
+            nstep[r * kNearMax + s] = NearStep{seg, (k * nyp + j) * nxp + i, q, loc};
+        }
+        nlen[r] = s;
+    } else if (id < V.n_rays + kNearRays + kNQ) {
+        const int q = id - V.n_rays - kNearRays;
+        pstep[q] = make_int4(q % 5 - 2, q / 5 % 5 - 2, q / 25 - 2, 0);
+        if (q == 0) kfirst[0] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Normalisation march (prc.cu:178-237 with NormOp): one wave = 64 consecutive sources of a row.
+// Per step the DDA offset is wave-uniform, so density reads are (uniform base + lane).
+
+struct Lane {
+    float w = 0.0f, surv = 1.0f, air = 0.0f;
+    bool alive;
+};
+
+// One DDA step of prc.cu:146-163 for the current lane; returns the deposit (0 if none).
+template <bool EXACT>
+__device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_here, const Vol& V,
+                                   const float* tab, unsigned long long& visits) {
+    float dep = 0.0f;
+    if (!(rho < INFINITY)) {  // the halo: the ray has left the grid (prc.cu:163)
+        st.alive = false;
+        return 0.0f;
+    }
+    ++visits;
+    const float wn = fminf(fmaf(rho, seg, st.w), V.cutoff);
+    const float sn = EXACT ? survival(V.br, V.nb, wn) : lookup(tab, V.scale, wn);
+    if (rho >= V.support_rho) {
+        st.air = 0.0f;
+        if (deposit_here && st.surv > sn) dep = st.surv - sn;
+    } else if ((st.air += seg) > V.air_cap) {
+        st.alive = false;
+        return 0.0f;
+    }
+    if (wn >= V.cutoff) st.alive = false;
+    st.w = wn, st.surv = sn;
+    return dep;
+}
+
+__device__ inline void load_table(float* tab, const Vol& V) {
+    for (int j = threadIdx.x; j <= kBins; j += blockDim.x) tab[j] = gload(V.table + j);
+    __syncthreads();
+}
+
+// Near block: 2048 rays per source, 125 bins accumulated in registers (bin index is uniform).
+template <bool EXACT, class WT>
+__global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                               int n_items, unsigned long long* vrs) {
+    __shared__ float tab[kBins + 1];
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    const int4 it = items[min(item_id, n_items - 1)];
+    const Vol& V = vols[it.x];
+    if (!EXACT) load_table(tab, V);
+    if (item_id >= n_items || it.w <= 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const bool act = valid && gload(V.support + loc);
+    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
+    unsigned long long visits = 0;
+
+    // Shell accumulators (shell 4 split in two halves of 18 to stay within register indexing).
+    float a0 = 0.0f, a1[6], a2[18], a3[32], a4l[18], a4h[18], a5[24], a6[8];
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        if (i < 6) a1[i] = 0.0f;
+        if (i < 18) a2[i] = a4l[i] = a4h[i] = 0.0f;
+        a3[i] = 0.0f;
+        if (i < 24) a5[i] = 0.0f;
+        if (i < 8) a6[i] = 0.0f;
+    }
+    for (int r = 0; r < kNearRays; ++r) {
+        const int len = gload(V.nlen + r);
+        const NearStep* nsr = V.nstep + r * kNearMax;
+        Lane st;
+        st.alive = act;
+        auto step = [&](int s, int& loc) {
+            const NearStep ns = gload(nsr + s);
+            loc = ns.loc;
+            return st.alive ? march_step<EXACT>(st, gload(rho0 + ns.dp + lane), ns.seg, true, V, tab, visits) : 0.0f;
+        };
+        int loc;
+        a0 += step(0, loc);  // every near ray starts inside the source voxel
+        if (len > 1) { const float d = step(1, loc); a1[loc] += d; }
+        if (len > 2) { const float d = step(2, loc); a2[loc] += d; }
+        if (len > 3) { const float d = step(3, loc); a3[loc] += d; }
+        if (len > 4) {  // both halves are updated, branch-free, to keep the registers in place
+            const float d = step(4, loc);
+            const bool lo = loc < 18;
+            a4l[lo ? loc : 0] += lo ? d : 0.0f;
+            a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
+        }
+        if (len > 5) { const float d = step(5, loc); a5[loc] += d; }
+        if (len > 6) { const float d = step(6, loc); a6[loc] += d; }
+    }
+    auto bin = [&](int q) {
+        const int sh = kShellMap.shell[q], lc = kShellMap.local[q];
+        return sh == 0 ? a0 : sh == 1 ? a1[lc] : sh == 2 ? a2[lc] : sh == 3 ? a3[lc]
+             : sh == 4 ? (lc < 18 ? a4l[lc] : a4h[lc - 18]) : sh == 5 ? a5[lc] : a6[lc];
+    };
+    // Near weights on the scale of one far ray (prc.cu:205-218); W starts with their sum.
+    const RowInfo ri = gload(V.rinfo + (size_t)row * (V.n_rays + 1));
+    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
+    WT* wp = (WT*)V.wbuf + ri.off + (x0 - xa);
+    const float nscale = (float)V.n_rays / (float)kNearRays;
+    float wacc = 0.0f;
+#pragma unroll
+    for (int q = 0; q < kNQ; ++q) {
+        const WT h = WeightIO<WT>::store_value(act ? bin(q) * nscale : 0.0f);
+        const float wq = WeightIO<WT>::to_float(h);
+        if (wq > 0.0f) wacc += wq;
+        if (valid) gstore(wp + (size_t)q * len + lane, h);
+    }
+    if (act) gstore(V.W + loc, wacc);
+    if (vrs) {
+        const long long tot = wave_sum_ll((long long)visits);
+        if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
+    }
+}
+
+// Far rays. PASS 1 counts the stored steps of every (row, ray); PASS 2 writes the weights and
+// finishes W (continuing from the near sum, in the reference's accumulation order).
+template <int PASS, bool EXACT, class WT>
+__global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                              int n_items, unsigned long long* vrs, unsigned long long* ndep) {
+    __shared__ float tab[kBins + 1];
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    const int4 it = items[min(item_id, n_items - 1)];
+    const Vol& V = vols[it.x];
+    if (!EXACT) load_table(tab, V);
+    if (item_id >= n_items || it.w <= 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const bool act = valid && gload(V.support + loc);
+    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
+    const int R1 = V.n_rays + 1;
+    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
+    unsigned long long visits = 0, deposits = 0;
+    float wacc = (PASS == 2 && act) ? gload(V.W + loc) : 0.0f;
+
+    for (int r = 0; r < V.n_rays; ++r) {
+        const int kf = gload(V.kfirst + 1 + r);
+        const FarStep* fs = V.fstep + (size_t)r * V.L;
+        int cnt = 0;
+        WT* wp = nullptr;
+        if (PASS == 2) {
+            const RowInfo ri = gload(V.rinfo + (size_t)row * R1 + 1 + r);
+            cnt = ri.cnt;
+            wp = (WT*)V.wbuf + ri.off + (x0 - xa) - (size_t)kf * len;
+        }
+        Lane st;
+        st.alive = act;
+        int last = -1;
+        int s = 0;
+        for (; s < V.L; ++s) {
+            const FarStep f = gload(fs + s);
+            float dep = 0.0f;
+            if (st.alive) dep = march_step<EXACT>(st, gload(rho0 + f.dp + lane), f.seg, s >= kf, V, tab, visits);
+            if (PASS == 1) {
+                if (dep > 0.0f) last = s;
+            } else if (s >= kf && s - kf < cnt) {
+                const WT h = WeightIO<WT>::store_value(dep);
+                const float wv = WeightIO<WT>::to_float(h);
+                if (wv > 0.0f) wacc += wv, ++deposits;
+                if (valid) gstore(wp + (size_t)s * len + lane, h);
+            }
+            if (!__any(st.alive)) {
+                ++s;
+                break;
+            }
+        }
+        if (PASS == 1) {
+            const int kmax = wave_max(last + 1);
+            if (lane == 0 && kmax > kf) atomicMax(&V.rinfo[(size_t)row * R1 + 1 + r].cnt, kmax - kf);
+        } else {
+            for (s = max(s, kf); s < kf + cnt; ++s)
+                if (valid) gstore(wp + (size_t)s * len + lane, WeightIO<WT>::store_value(0.0f));
+        }
+    }
+    if (PASS == 2 && act) gstore(V.W + loc, wacc);
+    if (PASS == 1 && vrs) {
+        const long long tot = wave_sum_ll((long long)visits);
+        if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
+    }
+    if (PASS == 2 && ndep) {
+        const long long tot = wave_sum_ll((long long)deposits);
+        if (lane == 0 && tot) atomicAdd(ndep, (unsigned long long)tot);
+    }
+}
+
+// Forward bound per (row, ray): the last step at which some source row reaches this row.
+__global__ void k_forward_bounds(Vol V, const int* kglob) {
+    const int R1 = V.n_rays + 1;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)V.ny * V.nz * R1) return;
+    const int row = (int)(id / R1), p = (int)(id % R1);
+    const int len = gload(V.row_len + row);
+    int kt = 0;
+    if (len > 0) {
+        const int xa = gload(V.row_xa + row), y = row % V.ny, z = row / V.ny, kf = gload(V.kfirst + p);
+        const int kend = gload(kglob + p);
+        for (int k = kf; k < kend; ++k) {
+            const int4 d = gload(V.pstep + (size_t)p * V.LP + k);
+            const int ys = y - d.y, zs = z - d.z;
+            if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
+            const int srow = zs * V.ny + ys;
+            const int slen = gload(V.row_len + srow);
+            if (slen == 0 || k - kf >= gload(V.rinfo + (size_t)srow * R1 + p).cnt) continue;
+            const int sxa = gload(V.row_xa + srow);
+            if (sxa + d.x < xa + len && xa < sxa + slen + d.x) kt = k + 1;
+        }
+    }
+    V.rinfo[id].kt = kt;
+}
+
+// ---------------------------------------------------------------------------------------
+// Projector passes.
+
+// g = f / W on active sources with f / W > 0 (ForwardOp::begin, prc.cu:244-248), else 0.
+__device__ inline float forward_scale(float f, float w) {
+    const float s = w > 0.0f ? f / w : 0.0f;
+    return s > 0.0f ? s : 0.0f;
+}
+
+// out = A g where g already holds f / W (prc.cu:239-254 as a gather). Every voxel of the
+// item is written (zeros where nothing arrives), so out needs no clearing.
+template <class WT>
+__global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                  int n_items, const float* __restrict__ gin,
+                                                  float* __restrict__ out) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    if (it.w <= 0) return;
+    const Vol& V = vols[it.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    float* o = out + V.vbase + (size_t)row * V.nx + x0;
+    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
+    if (len == 0 || x0 + it.w <= xa || x0 >= xa + len) {
+        if (lane < it.w) o[lane] = 0.0f;
+        return;
+    }
+    const float* g = gin + V.vbase;
+    const WT* wb = (const WT*)V.wbuf;
+    const int R1 = V.n_rays + 1;
+    float acc = 0.0f;
+    for (int p = 0; p < R1; ++p) {
+        const int kf = gload(V.kfirst + p);
+        const int kend = gload(V.rinfo + (size_t)row * R1 + p).kt;
+        const int4* ps = V.pstep + (size_t)p * V.LP;
+        for (int k = kf; k < kend; ++k) {
+            const int4 d = gload(ps + k);
+            const int ys = y - d.y, zs = z - d.z;
+            if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
+            const int srow = zs * V.ny + ys;
+            const RowInfo ri = gload(V.rinfo + (size_t)srow * R1 + p);
+            if (k - kf >= ri.cnt) continue;
+            const int sxa = gload(V.row_xa + srow), slen = gload(V.row_len + srow);
+            const int xs = x0 - d.x + lane;
+            if ((unsigned)(xs - sxa) < (unsigned)slen) {
+                const float w = WeightIO<WT>::load(wb + ri.off + (size_t)(k - kf) * slen + (xs - sxa));
+                if (w != 0.0f) acc = fmaf(w, gload(g + (size_t)srow * V.nx + xs), acc);
+            }
+        }
+    }
+    if (lane < it.w) o[lane] = acc;
+}
+
+// out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the planes.
+template <class WT>
+__global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                  int n_items, const float* __restrict__ in,
+                                                  float* __restrict__ out) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    if (it.w <= 0) return;
+    const Vol& V = vols[it.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
+    const float* src = in + V.vbase;
+    const WT* wb = (const WT*)V.wbuf;
+    const int R1 = V.n_rays + 1;
+    float acc = 0.0f;
+    for (int p = 0; p < R1; ++p) {
+        const RowInfo ri = gload(V.rinfo + (size_t)row * R1 + p);
+        const int kf = gload(V.kfirst + p);
+        const WT* wp = wb + ri.off + (x0 - xa);
+        const int4* ps = V.pstep + (size_t)p * V.LP + kf;
+        for (int j = 0; j < ri.cnt; ++j) {
+            const int4 d = gload(ps + j);
+            if (valid) {
+                const float w = WeightIO<WT>::load(wp + (size_t)j * len + lane);
+                if (w != 0.0f)
+                    acc = fmaf(w, gload(src + ((size_t)(z + d.z) * V.ny + (y + d.y)) * V.nx + x0 + d.x + lane), acc);
+            }
+        }
+    }
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    if (valid && gload(V.support + loc)) {
+        const float wv = gload(V.W + loc);
+        out[V.vbase + loc] = wv > 0.0f ? acc / wv : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Image updates (prc.cu:269-332), one thread per voxel over row-segment items.
+
+#define ITEM_PROLOGUE                                                          \
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();    \
+    if (item_id >= n_items) return;                                            \
+    const int4 it = items[item_id];                                            \
+    const int lane = threadIdx.x & (kWave - 1);                                \
+    if (lane >= it.w) return;                                                  \
+    const Vol& V = vols[it.x];                                                 \
+    const size_t loc = (size_t)it.y * V.nx + it.z + lane;                      \
+    const size_t i = V.vbase + loc;                                            \
+    (void)i;
+
+// g = f / W for the forward (ForwardOp::begin).
+__global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                         const float* __restrict__ f, float* __restrict__ g) {
+    ITEM_PROLOGUE
+    g[i] = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
+}
+
+// Biggs-Andrews prediction y = max(f + lambda (f - prev), f / 2) (prc.cu:318-321), fused with
+// the forward scaling of y.
+__global__ void k_extrapolate(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                              const Scal* __restrict__ sc, const float* __restrict__ f,
+                              const float* __restrict__ prev, float* __restrict__ y, float* __restrict__ g) {
+    ITEM_PROLOGUE
+    const float lambda = sc[it.x].lambda;
+    const float fi = f[i];
+    const float yi = fmaxf(fmaf(lambda, fi - prev[i], fi), 0.5f * fi);
+    y[i] = yi;
+    g[i] = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
+}
+
+// ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place.
+__global__ void k_ratio(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                        const Scal* __restrict__ sc, const float* __restrict__ data, float* __restrict__ est) {
+    ITEM_PROLOGUE
+    est[i] = gload(V.support + loc) ? data[i] / fmaxf(est[i], sc[it.x].eps) : 0.0f;
+}
+
+// RDP MAP-EM surrogate update (prc.cu:279-309). Writes next and per-item FP64 partial sums.
+__global__ void k_update(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                         const Scal* __restrict__ sc, const float* __restrict__ f, const float* __restrict__ back,
+                         float* __restrict__ next, double* __restrict__ partial) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    const Scal& S = sc[it.x];
+    float result = 0.0f;
+    if (lane < it.w) {
+        const int row = it.y, xi = it.z + lane, yi = row % V.ny, zi = row / V.ny;
+        const size_t loc = (size_t)row * V.nx + xi;
+        const size_t idx = V.vbase + loc;
+        if (gload(V.support + loc)) {
+            const float fj = f[idx], f_em = fj * back[idx];
+            if (S.beta == 0.0f) {
+                result = fmaxf(f_em, 0.0f);
+            } else {
+                const float gamma = S.gamma, eps = S.eps;
+                const float smin = fminf(V.sx, fminf(V.sy, V.sz));
+                float grad = 0.0f, curv = 0.0f;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            if (!(dx | dy | dz)) continue;
+                            const int a = xi + dx, b = yi + dy, c = zi + dz;
+                            if (a < 0 || a >= V.nx || b < 0 || b >= V.ny || c < 0 || c >= V.nz) continue;
+                            const long nb = (long)loc + ((long)dz * V.ny + dy) * V.nx + dx;
+                            if (!gload(V.support + nb)) continue;
+                            const float mx = dx * V.sx, my = dy * V.sy, mz = dz * V.sz;
+                            const float w = smin / sqrtf(mx * mx + my * my + mz * mz);
+                            const float fk = f[V.vbase + nb], d = fj - fk, ad = fabsf(d);
+                            const float D = fj + fk + gamma * ad + eps, invD = 1.0f / D;
+                            grad += w * d * (fj + 3.0f * fk + gamma * ad + 2.0f * eps) * invD * invD;
+                            const float half_quad = (2.0f * (fj + fk + eps) + gamma * ad) * invD * invD;
+                            const float hess = 2.0f * (2.0f * fk + eps) * (2.0f * fk + eps) * invD * invD * invD;
+                            curv += 2.0f * w * fmaxf(half_quad, hess);
+                        }
+                const float a = S.beta * curv, B = 1.0f + S.beta * (grad - curv * fj);
+                const float root = sqrtf(B * B + 4.0f * a * f_em);
+                result = B >= 0.0f ? (f_em > 0.0f ? 2.0f * f_em / (B + root) : 0.0f) : (root - B) / (2.0f * a);
+            }
+        }
+        next[idx] = result;
+    }
+    const double tot = wave_sum_d((double)result);
+    if (lane == 0) partial[item_id] = tot;
+}
+
+// f = max(f, floor) on support (prc.cu:311-314) and per-item FP64 partial sums.
+__global__ void k_initialize(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                             const Scal* __restrict__ sc, float* __restrict__ f, double* __restrict__ partial) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    float v = 0.0f;
+    if (lane < it.w) {
+        const size_t loc = (size_t)it.y * V.nx + it.z + lane, i = V.vbase + loc;
+        v = f[i];
+        if (gload(V.support + loc)) v = fmaxf(v, 1000.0f * sc[it.x].eps);
+        f[i] = v;
+    }
+    const double tot = wave_sum_d((double)v);
+    if (lane == 0) partial[item_id] = tot;
+}
+
+// next = RN(next * scale); with acceleration residual = next - residual (prc.cu:325-332), plus
+// per-item partials of <h, h_prev> and <h_prev, h_prev>.
+__global__ void k_finish(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                         const Scal* __restrict__ sc, float* __restrict__ next, float* __restrict__ residual,
+                         const float* __restrict__ hprev, double* __restrict__ partial, int use_scale) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    double hy = 0.0, hh = 0.0;
+    if (lane < it.w) {
+        const size_t i = V.vbase + (size_t)it.y * V.nx + it.z + lane;
+        const float value = next[i] * (use_scale ? sc[it.x].scale : 1.0f);
+        next[i] = value;
+        if (residual) {
+            const float h = value - residual[i];
+            residual[i] = h;
+            const double hp = hprev[i];
+            hy = (double)h * hp;
+            hh = hp * hp;
+        }
+    }
+    if (partial) {
+        hy = wave_sum_d(hy);
+        hh = wave_sum_d(hh);
+        if (lane == 0) partial[2 * item_id] = hy, partial[2 * item_id + 1] = hh;
+    }
+}
+
+// Per-volume FP64 totals of the item partials, fixed order (one block per volume).
+// mode 0: initial count normalisation  scale = support_sum / sum
+// mode 1: prior count normalisation    scale = sum > 0 ? support_sum / sum : 1 (beta > 0 only)
+// mode 2: Biggs-Andrews lambda from (hy, hh) pairs
+// mode 3: plain sum into Scal::sum
+__global__ void k_reduce_volume(const int* __restrict__ item_begin, const double* __restrict__ partial,
+                                Scal* __restrict__ sc, int mode) {
+    __shared__ double sh[2][256];
+    const int v = blockIdx.x;
+    const int b = item_begin[v], e = item_begin[v + 1];
+    const int stride = mode == 2 ? 2 : 1;
+    double a0 = 0.0, a1 = 0.0;
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+        a0 += partial[(size_t)stride * i];
+        if (mode == 2) a1 += partial[(size_t)stride * i + 1];
+    }
+    sh[0][threadIdx.x] = a0, sh[1][threadIdx.x] = a1;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[0][threadIdx.x] += sh[0][threadIdx.x + o], sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x) return;
+    Scal& S = sc[v];
+    const double s0 = sh[0][0], s1 = sh[1][0];
+    if (mode == 0) {
+        S.sum = s0;
+        S.scale = (float)(S.support_sum / s0);
+    } else if (mode == 1) {
+        S.sum = s0;
+        S.scale = (S.beta > 0.0f && s0 > 0.0) ? (float)(S.support_sum / s0) : 1.0f;
+    } else if (mode == 2) {
+        S.hy = s0, S.hh = s1;
+        S.lambda = s1 > 0.0 ? (float)fmin(1.0, fmax(0.0, s0 / s1)) : 0.0f;
+    } else {
+        S.sum = s0;
+    }
+}
+
+__global__ void k_sum_partial(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                              const float* __restrict__ a, double* __restrict__ partial) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    double v = 0.0;
+    if (lane < it.w) v = a[V.vbase + (size_t)it.y * V.nx + it.z + lane];
+    v = wave_sum_d(v);
+    if (lane == 0) partial[item_id] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side.
+
+thread_local std::string g_error;
+
+struct Failure {
+    int code;
+};
+
+[[noreturn]] void fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    throw Failure{code};
+}
+
+#define HIPCHK(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) fail(PRC_ERR_DEVICE, "HIP error '%s' at %s:%d", hipGetErrorString(e_), \
+                                   __FILE__, __LINE__);                                           \
+    } while (0)
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return PRC_OK;
+    } catch (const Failure& e) {
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "host allocation failed";
+        return PRC_ERR_MEMORY;
+    }
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    T* p = nullptr;
+    if (n == 0) n = 1;
+    const hipError_t e = hipMalloc((void**)&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        fail(PRC_ERR_MEMORY, "device allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    }
+    return p;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+inline unsigned blocks_for(long long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// One volume's projector.
+struct prc_projector {
+    prc_geometry g{};
+    Isotope model{};
+    Vol hv{};
+    Vol* dv = nullptr;
+    float* rho_p = nullptr;
+    float* table = nullptr;
+    unsigned char* support = nullptr;
+    float* W = nullptr;
+    int* row_xa = nullptr;
+    int* row_len = nullptr;
+    RowInfo* rinfo = nullptr;
+    void* wbuf = nullptr;
+    FarStep* fstep = nullptr;
+    int4* pstep = nullptr;
+    int* kfirst = nullptr;
+    NearStep* nstep = nullptr;
+    int* nlen = nullptr;
+    int4* items_act = nullptr;  // active row segments (march, adjoint)
+    int4* items_all = nullptr;  // every row segment (forward, streaming)
+    int n_act = 0, n_all = 0;
+    std::vector<int4> h_items_act, h_items_all;
+    std::vector<unsigned char> h_support;
+    unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
+    float* scratch_in = nullptr;
+    float* scratch_out = nullptr;
+    prc_stats stats{};
+    size_t n = 0;
+    int wbytes = 4;
+};
+
+struct prc_solver {
+    prc_projector* A = nullptr;
+    prc_solve_params p{};
+    float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
+          *gf = nullptr;
+    Scal* sc = nullptr;
+    double* partial = nullptr;
+    int* item_begin = nullptr;
+    double support_sum = 0.0;
+    long long n_active = 0;
+    bool initialised = false;
+    int iterations_done = 0;
+};
+
+namespace {
+
+// Row-segment work items: 64 consecutive voxels of one X row. Lists are padded to a multiple of
+// kItemsPerBlock with empty items so that a block never mixes volumes.
+void build_items(prc_projector* A) {
+    const int nx = A->g.nx, rows = A->g.ny * A->g.nz;
+    A->h_items_act.clear();
+    A->h_items_all.clear();
+    std::vector<int> xa(rows, 0), len(rows, 0);
+    for (int row = 0; row < rows; ++row) {
+        const unsigned char* s = A->h_support.data() + (size_t)row * nx;
+        int lo = -1, hi = -1;
+        for (int x = 0; x < nx; ++x)
+            if (s[x]) {
+                if (lo < 0) lo = x;
+                hi = x;
+            }
+        if (lo >= 0) {
+            xa[row] = lo, len[row] = hi - lo + 1;
+            for (int x0 = lo; x0 <= hi; x0 += kWave) A->h_items_act.push_back(make_int4(0, row, x0, std::min(kWave, hi + 1 - x0)));
+        }
+        for (int x0 = 0; x0 < nx; x0 += kWave) A->h_items_all.push_back(make_int4(0, row, x0, std::min(kWave, nx - x0)));
+    }
+    while (A->h_items_act.size() % kItemsPerBlock) A->h_items_act.push_back(make_int4(0, 0, 0, 0));
+    while (A->h_items_all.size() % kItemsPerBlock) A->h_items_all.push_back(make_int4(0, 0, 0, 0));
+    A->n_act = (int)A->h_items_act.size();
+    A->n_all = (int)A->h_items_all.size();
+    A->row_xa = dalloc<int>(rows);
+    A->row_len = dalloc<int>(rows);
+    HIPCHK(hipMemcpy(A->row_xa, xa.data(), rows * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(A->row_len, len.data(), rows * sizeof(int), hipMemcpyHostToDevice));
+    A->items_act = dalloc<int4>(A->n_act);
+    A->items_all = dalloc<int4>(A->n_all);
+    HIPCHK(hipMemcpy(A->items_act, A->h_items_act.data(), A->n_act * sizeof(int4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(A->items_all, A->h_items_all.data(), A->n_all * sizeof(int4), hipMemcpyHostToDevice));
+    long long n_rows = 0;
+    for (int row = 0; row < rows; ++row) n_rows += len[row] > 0;
+    A->stats.n_rows = n_rows;
+}
+
+template <class WT>
+void launch_far(prc_projector* A, int pass) {
+    const bool exact = A->hv.exact;
+    const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
+    if (pass == 1) {
+        if (exact) k_far<1, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
+        else k_far<1, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
+    } else {
+        if (exact) k_far<2, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
+        else k_far<2, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+template <class WT>
+void launch_near(prc_projector* A) {
+    const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
+    if (A->hv.exact) k_near<true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+    else k_near<false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+    HIPCHK(hipGetLastError());
+}
+
+void build_projector(prc_projector* A, const float* host_rho, const float* host_table) {
+    const prc_geometry& g = A->g;
+    const double t0 = now_s();
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    A->n = n;
+    A->model = renormalised(g.isotope);
+    const float cutoff = range_cutoff_host(A->model, 0.9995f);  // prc.cu:587
+    A->stats.cutoff_mm = cutoff;
+    A->stats.n_voxels = (long long)n;
+    A->wbytes = g.precision == PRC_PRECISION_MIXED ? 2 : 4;
+
+    // Density: clamp (prc.cu:603), support mask (prc.cu:605), +inf halo of width one voxel.
+    const int nxp = g.nx + 2, nyp = g.ny + 2, nzp = g.nz + 2;
+    const size_t np = (size_t)nxp * nyp * nzp;
+    std::vector<float> rp(np, INFINITY);
+    A->h_support.assign(n, 0);
+    long long n_active = 0;
+    for (int k = 0; k < g.nz; ++k)
+        for (int j = 0; j < g.ny; ++j) {
+            const size_t src = ((size_t)k * g.ny + j) * g.nx;
+            float* dst = rp.data() + (((size_t)(k + 1) * nyp + (j + 1)) * nxp + 1);
+            for (int i = 0; i < g.nx; ++i) {
+                const float r = std::max(host_rho[src + i], kMinDensity);
+                dst[i] = r;
+                const bool s = r >= g.support_rho;
+                A->h_support[src + i] = s;
+                n_active += s;
+            }
+        }
+    A->stats.n_active = n_active;
+    A->rho_p = dalloc<float>(np);
+    HIPCHK(hipMemcpy(A->rho_p, rp.data(), np * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<float>().swap(rp);
+    A->support = dalloc<unsigned char>(n);
+    HIPCHK(hipMemcpy(A->support, A->h_support.data(), n, hipMemcpyHostToDevice));
+    A->W = dalloc<float>(n);
+    HIPCHK(hipMemset(A->W, 0, n * sizeof(float)));
+    A->counters = dalloc<unsigned long long>(4);
+    HIPCHK(hipMemset(A->counters, 0, 4 * sizeof(unsigned long long)));
+
+    // Range table on the device (prc.cu:625-633) unless the caller supplies one.
+    const float scale = (float)kBins / cutoff;
+    A->table = dalloc<float>(kBins + 1);
+    if (host_table) {
+        HIPCHK(hipMemcpy(A->table, host_table, (kBins + 1) * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        Branch b[3] = {};
+        for (int i = 0; i < A->model.n; ++i) b[i] = A->model.br[i];
+        k_range_table<<<blocks_for(kBins + 1, 256), 256>>>(A->table, b[0], b[1], b[2], A->model.n, scale);
+        HIPCHK(hipGetLastError());
+    }
+
+    // Ray and step tables.
+    const int R = g.n_rays, R1 = R + 1;
+    const int L = g.nx + g.ny + g.nz;  // a ray leaves any grid within nx+ny+nz voxel visits
+    const int LP = std::max(L, kNQ);
+    A->fstep = dalloc<FarStep>((size_t)R * L);
+    A->pstep = dalloc<int4>((size_t)R1 * LP);
+    A->kfirst = dalloc<int>(R1);
+    A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
+    A->nlen = dalloc<int>(kNearRays);
+    HIPCHK(hipMemset(A->nstep, 0, (size_t)kNearRays * kNearMax * sizeof(NearStep)));
+
+    Vol& V = A->hv;
+    V.nx = g.nx, V.ny = g.ny, V.nz = g.nz, V.n_rays = R;
+    V.sx = g.sx, V.sy = g.sy, V.sz = g.sz, V.cutoff = cutoff, V.support_rho = g.support_rho;
+    V.air_cap = g.air_cap_mm, V.scale = scale;
+    V.nb = A->model.n;
+    for (int i = 0; i < 3; ++i) V.br[i] = i < A->model.n ? A->model.br[i] : Branch{0, 0, 0, 0, 0};
+    V.exact = (g.flags & PRC_FLAG_EXACT_RANGE) != 0;
+    V.L = L, V.LP = LP, V.vbase = 0;
+    V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
+    V.fstep = A->fstep, V.pstep = A->pstep, V.kfirst = A->kfirst, V.nstep = A->nstep, V.nlen = A->nlen;
+
+    k_rays<<<blocks_for(R + kNearRays + kNQ, 128), 128>>>(V, A->fstep, A->pstep, A->kfirst, A->nstep, A->nlen);
+    HIPCHK(hipGetLastError());
+
+    build_items(A);
+    V.row_xa = A->row_xa, V.row_len = A->row_len;
+    const size_t rows = (size_t)g.ny * g.nz;
+    A->rinfo = dalloc<RowInfo>(rows * R1);
+    HIPCHK(hipMemset(A->rinfo, 0, rows * R1 * sizeof(RowInfo)));
+    V.rinfo = A->rinfo;
+    V.wbuf = nullptr;
+    A->dv = dalloc<Vol>(1);
+    HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
+
+    // Pass 1: count the stored steps of every (row, far ray).
+    if (A->wbytes == 2) launch_far<__half>(A, 1);
+    else launch_far<float>(A, 1);
+
+    // Offsets of the row planes (host prefix sum), then the weight buffer.
+    std::vector<RowInfo> ri(rows * R1);
+    HIPCHK(hipMemcpy(ri.data(), A->rinfo, rows * R1 * sizeof(RowInfo), hipMemcpyDeviceToHost));
+    std::vector<int> h_kfirst(R1);
+    HIPCHK(hipMemcpy(h_kfirst.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> len(rows);
+    HIPCHK(hipMemcpy(len.data(), A->row_len, rows * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> kglob(R1, 0);
+    long long off = 0;
+    for (size_t row = 0; row < rows; ++row) {
+        for (int p = 0; p < R1; ++p) {
+            RowInfo& e = ri[row * R1 + p];
+            if (len[row] == 0) {
+                e.cnt = 0;
+            } else if (p == 0) {
+                e.cnt = kNQ;
+            }
+            e.off = off;
+            off += (long long)e.cnt * len[row];
+            kglob[p] = std::max(kglob[p], h_kfirst[p] + e.cnt);
+        }
+    }
+    A->stats.weight_entries = off;
+    A->stats.weight_bytes = off * A->wbytes;
+    HIPCHK(hipMemcpy(A->rinfo, ri.data(), rows * R1 * sizeof(RowInfo), hipMemcpyHostToDevice));
+    A->wbuf = (void*)dalloc<unsigned char>((size_t)std::max<long long>(off, 1) * A->wbytes);
+    V.wbuf = A->wbuf;
+    HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
+
+    int* d_kglob = dalloc<int>(R1);
+    HIPCHK(hipMemcpy(d_kglob, kglob.data(), R1 * sizeof(int), hipMemcpyHostToDevice));
+    k_forward_bounds<<<blocks_for((long long)rows * R1, 256), 256>>>(V, d_kglob);
+    HIPCHK(hipGetLastError());
+
+    // Pass 2: near block, then far rays, writing every weight and W.
+    if (A->wbytes == 2) {
+        launch_near<__half>(A);
+        launch_far<__half>(A, 2);
+    } else {
+        launch_near<float>(A);
+        launch_far<float>(A, 2);
+    }
+    HIPCHK(hipDeviceSynchronize());
+    dfree(d_kglob);
+    unsigned long long c[4];
+    HIPCHK(hipMemcpy(c, A->counters, sizeof(c), hipMemcpyDeviceToHost));
+    A->stats.vrs_near = (long long)c[0];
+    A->stats.vrs_far = (long long)c[1];
+    A->stats.deposits = (long long)c[2];
+    A->stats.norm_seconds = now_s() - t0;
+}
+
+void release(prc_projector* A) {
+    dfree(A->dv), dfree(A->rho_p), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
+    dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
+    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
+    dfree(A->scratch_out);
+    if (A->wbuf) (void)hipFree(A->wbuf);
+    A->wbuf = nullptr;
+}
+
+// Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
+void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out) {
+    const unsigned nb = blocks_for(n_items, kItemsPerBlock);
+    if (wbytes == 2) k_forward<__half><<<nb, 256>>>(dv, items, n_items, g, out);
+    else k_forward<float><<<nb, 256>>>(dv, items, n_items, g, out);
+    HIPCHK(hipGetLastError());
+}
+
+void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out) {
+    const unsigned nb = blocks_for(n_items, kItemsPerBlock);
+    if (wbytes == 2) k_adjoint<__half><<<nb, 256>>>(dv, items, n_items, in, out);
+    else k_adjoint<float><<<nb, 256>>>(dv, items, n_items, in, out);
+    HIPCHK(hipGetLastError());
+}
+
+void check_geometry(const prc_geometry* g) {
+    if (!g) fail(PRC_ERR_INVALID, "null geometry");
+    if (g->nx <= 0 || g->ny <= 0 || g->nz <= 0 || !(g->sx > 0) || !(g->sy > 0) || !(g->sz > 0) ||
+        !std::isfinite(g->sx) || !std::isfinite(g->sy) || !std::isfinite(g->sz) || g->n_rays < 2 || (g->n_rays & 1) ||
+        g->n_rays > INT_MAX - 31 || !(g->support_rho >= 0) || !std::isfinite(g->support_rho) ||
+        !(g->air_cap_mm > 0) || !std::isfinite(g->air_cap_mm) || g->isotope < 0 || g->isotope >= 8 ||
+        (g->precision != PRC_PRECISION_FP32 && g->precision != PRC_PRECISION_MIXED))
+        fail(PRC_ERR_INVALID, "Invalid geometry or solver parameters");
+    if (g->nx > INT_MAX / g->ny || g->nx * g->ny > INT_MAX / g->nz)
+        fail(PRC_ERR_INVALID, "Volume exceeds the INT_MAX voxel index limit");
+    if (g->nx + g->ny + g->nz > (1 << 24)) fail(PRC_ERR_INVALID, "grid too long for the ray tables");
+}
+
+// ---------------------------------------------------------------------------------------
+// Solver (prc.cu:659-710), all volumes of a batch at once. Buffers are concatenated voxel
+// vectors; S (host) and sc (device) hold the per-volume scalars.
+struct Batch {
+    const Vol* dv = nullptr;
+    const int4* items_all = nullptr;
+    const int4* items_act = nullptr;
+    int n_all = 0, n_act = 0, n_vols = 1, wbytes = 4;
+    const int* item_begin = nullptr;  // per-volume ranges of items_all
+    float *g, *f, *prev, *y, *hprev, *est, *gf;
+    Scal* sc;
+    double* partial;
+    bool accel = false, any_prior = false;
+    hipStream_t stream = 0;
+};
+
+void batch_initialise(const Batch& B) {
+    const unsigned nb = blocks_for(B.n_all, kItemsPerBlock);
+    k_initialize<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.partial);
+    k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 0);
+    k_finish<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, nullptr, nullptr, nullptr, 1);
+    HIPCHK(hipGetLastError());
+}
+
+// One iteration; the caller rotates the buffers with batch_rotate afterwards.
+void batch_iteration(const Batch& B) {
+    const unsigned nb = blocks_for(B.n_all, kItemsPerBlock);
+    const unsigned na = blocks_for(B.n_act, kItemsPerBlock);
+    const float* start = B.f;
+    if (B.accel) {
+        k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf);
+        start = B.y;
+    } else {
+        k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
+    }
+    if (B.wbytes == 2) k_forward<__half><<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.gf, B.est);
+    else k_forward<float><<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.gf, B.est);
+    k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
+    if (B.wbytes == 2) k_adjoint<__half><<<na, 256, 0, B.stream>>>(B.dv, B.items_act, B.n_act, B.est, B.prev);
+    else k_adjoint<float><<<na, 256, 0, B.stream>>>(B.dv, B.items_act, B.n_act, B.est, B.prev);
+    k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
+    if (B.any_prior) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 1);
+    if (B.any_prior || B.accel)
+        k_finish<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.est, B.accel ? B.y : nullptr,
+                                           B.hprev, B.accel ? B.partial : nullptr, B.any_prior ? 1 : 0);
+    if (B.accel) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 2);
+    HIPCHK(hipGetLastError());
+}
+
+void batch_rotate(Batch& B) {
+    if (B.accel) {
+        std::swap(B.y, B.hprev);
+        std::swap(B.prev, B.f);
+    }
+    std::swap(B.f, B.est);
+}
+
+}  // namespace
+
+// =======================================================================================
+extern "C" {
+
+const char* prc_version(void) { return "prc-mi355x 0.1 (gfx950)"; }
+const char* prc_last_error(void) { return g_error.c_str(); }
+int prc_isotope_count(void) { return 8; }
+const char* prc_isotope_name(int index) { return index >= 0 && index < 8 ? kIsotopes[index].name : nullptr; }
+int prc_isotope_find(const char* name) {
+    if (!name) return -1;
+    for (int i = 0; i < 8; ++i)
+        if (!strcmp(name, kIsotopes[i].name)) return i;
+    return -1;
+}
+float prc_range_cutoff(int isotope, float fraction) {
+    if (isotope < 0 || isotope >= 8) return NAN;
+    return range_cutoff_host(renormalised(isotope), fraction);
+}
+
+int prc_projector_create(const prc_geometry* geometry, const float* host_rho, const float* host_table,
+                         prc_projector** out) {
+    return guarded([&] {
+        if (!out || !host_rho) fail(PRC_ERR_INVALID, "null argument");
+        *out = nullptr;
+        check_geometry(geometry);
+        const size_t n = (size_t)geometry->nx * geometry->ny * geometry->nz;
+        for (size_t i = 0; i < n; ++i)
+            if (!std::isfinite(host_rho[i])) fail(PRC_ERR_INVALID, "Non-finite density at voxel %zu", i);
+        prc_projector* A = new prc_projector;
+        A->g = *geometry;
+        try {
+            build_projector(A, host_rho, host_table);
+        } catch (...) {
+            release(A);
+            delete A;
+            throw;
+        }
+        *out = A;
+    });
+}
+
+void prc_projector_destroy(prc_projector* A) {
+    if (!A) return;
+    release(A);
+    delete A;
+}
+
+int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out) {
+    return guarded([&] {
+        if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
+        if (!A->scratch_in) A->scratch_in = dalloc<float>(A->n);
+        k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, A->scratch_in);
+        HIPCHK(hipGetLastError());
+        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, A->scratch_in, dev_out);
+    });
+}
+
+int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out) {
+    return guarded([&] {
+        if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, dev_in, dev_out);
+    });
+}
+
+static int host_op(prc_projector* A, const float* in, float* out, bool fwd) {
+    return guarded([&] {
+        if (!A || !in || !out) fail(PRC_ERR_INVALID, "null argument");
+        float* din = dalloc<float>(A->n);
+        float* dout = nullptr;
+        try {
+            dout = dalloc<float>(A->n);
+            HIPCHK(hipMemcpy(din, in, A->n * sizeof(float), hipMemcpyHostToDevice));
+            if (fwd) {
+                const int rc = prc_projector_forward(A, din, dout);
+                if (rc) throw Failure{rc};
+            } else {
+                HIPCHK(hipMemcpy(dout, out, A->n * sizeof(float), hipMemcpyHostToDevice));
+                const int rc = prc_projector_adjoint(A, din, dout);
+                if (rc) throw Failure{rc};
+            }
+            HIPCHK(hipMemcpy(out, dout, A->n * sizeof(float), hipMemcpyDeviceToHost));
+        } catch (...) {
+            dfree(din), dfree(dout);
+            throw;
+        }
+        dfree(din), dfree(dout);
+    });
+}
+
+int prc_projector_forward_host(prc_projector* A, const float* in, float* out) { return host_op(A, in, out, true); }
+int prc_projector_adjoint_host(prc_projector* A, const float* in, float* out) { return host_op(A, in, out, false); }
+
+int prc_projector_norm(prc_projector* A, float* host_w) {
+    return guarded([&] {
+        if (!A || !host_w) fail(PRC_ERR_INVALID, "null argument");
+        HIPCHK(hipMemcpy(host_w, A->W, A->n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+const float* prc_projector_norm_device(prc_projector* A) { return A ? A->W : nullptr; }
+
+int prc_projector_table(prc_projector* A, float* host_table, float* scale) {
+    return guarded([&] {
+        if (!A || !host_table) fail(PRC_ERR_INVALID, "null argument");
+        HIPCHK(hipMemcpy(host_table, A->table, (kBins + 1) * sizeof(float), hipMemcpyDeviceToHost));
+        if (scale) *scale = A->hv.scale;
+    });
+}
+
+int prc_projector_stats(prc_projector* A, prc_stats* stats) {
+    return guarded([&] {
+        if (!A || !stats) fail(PRC_ERR_INVALID, "null argument");
+        *stats = A->stats;
+    });
+}
+
+int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_params* params, prc_solver** out) {
+    return guarded([&] {
+        if (!A || !host_g || !params || !out) fail(PRC_ERR_INVALID, "null argument");
+        *out = nullptr;
+        if (!(params->beta >= 0) || !(params->gamma >= 0) || !std::isfinite(params->beta) ||
+            !std::isfinite(params->gamma))
+            fail(PRC_ERR_INVALID, "Invalid geometry or solver parameters");
+        const size_t n = A->n;
+        std::vector<float> g(n);
+        double support_sum = 0.0;
+        for (size_t i = 0; i < n; ++i) {  // prc.cu:598-613
+            if (!std::isfinite(host_g[i])) fail(PRC_ERR_INVALID, "Non-finite density or activity at voxel %zu", i);
+            const float v = std::max(host_g[i], 0.0f);
+            g[i] = A->h_support[i] ? v : 0.0f;
+            if (A->h_support[i]) support_sum += v;
+        }
+        if (support_sum == 0.0) fail(PRC_ERR_NO_ACTIVITY, "No activity in voxels with density >= %g: nothing to do",
+                                     (double)A->g.support_rho);
+        prc_solver* S = new prc_solver;
+        S->A = A;
+        S->p = *params;
+        S->support_sum = support_sum;
+        S->n_active = A->stats.n_active;
+        try {
+            S->g = dalloc<float>(n), S->f = dalloc<float>(n), S->prev = dalloc<float>(n), S->est = dalloc<float>(n);
+            S->gf = dalloc<float>(n);
+            if (params->accelerate) S->y = dalloc<float>(n), S->hprev = dalloc<float>(n);
+            HIPCHK(hipMemcpy(S->g, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(S->f, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemset(S->prev, 0, n * sizeof(float)));
+            HIPCHK(hipMemset(S->est, 0, n * sizeof(float)));
+            if (S->y) HIPCHK(hipMemset(S->y, 0, n * sizeof(float)));
+            if (S->hprev) HIPCHK(hipMemset(S->hprev, 0, n * sizeof(float)));
+            Scal sc{};
+            sc.support_sum = support_sum;
+            sc.eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);  // prc.cu:668
+            sc.lambda = 0.0f, sc.scale = 1.0f, sc.beta = params->beta, sc.gamma = params->gamma;
+            sc.accel = params->accelerate ? 1 : 0;
+            S->sc = dalloc<Scal>(1);
+            HIPCHK(hipMemcpy(S->sc, &sc, sizeof(Scal), hipMemcpyHostToDevice));
+            S->partial = dalloc<double>(2 * (size_t)A->n_all);
+            int ib[2] = {0, A->n_all};
+            S->item_begin = dalloc<int>(2);
+            HIPCHK(hipMemcpy(S->item_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
+        } catch (...) {
+            dfree(S->g), dfree(S->f), dfree(S->prev), dfree(S->est), dfree(S->gf), dfree(S->y), dfree(S->hprev);
+            dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
+            delete S;
+            throw;
+        }
+        *out = S;
+    });
+}
+
+void prc_solver_destroy(prc_solver* S) {
+    if (!S) return;
+    dfree(S->g), dfree(S->f), dfree(S->prev), dfree(S->est), dfree(S->gf), dfree(S->y), dfree(S->hprev);
+    dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
+    delete S;
+}
+
+static Batch solver_batch(prc_solver* S) {
+    Batch B;
+    prc_projector* A = S->A;
+    B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
+    B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
+    B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
+    B.sc = S->sc, B.partial = S->partial;
+    B.accel = S->p.accelerate != 0, B.any_prior = S->p.beta > 0.0f;
+    return B;
+}
+
+int prc_solver_iterate(prc_solver* S, int n) {
+    return guarded([&] {
+        if (!S || n < 0) fail(PRC_ERR_INVALID, "invalid argument");
+        if (n == 0) return;
+        Batch B = solver_batch(S);
+        if (!S->initialised) {
+            batch_initialise(B);
+            S->initialised = true;
+        }
+        for (int it = 0; it < n; ++it) {
+            batch_iteration(B);
+            batch_rotate(B);
+        }
+        S->f = B.f, S->prev = B.prev, S->y = B.y, S->hprev = B.hprev, S->est = B.est;
+        S->iterations_done += n;
+    });
+}
+
+int prc_solver_sync(prc_solver* S) {
+    return guarded([&] {
+        if (!S) fail(PRC_ERR_INVALID, "null argument");
+        HIPCHK(hipDeviceSynchronize());
+    });
+}
+
+int prc_solver_estimate(prc_solver* S, float* host_out) {
+    return guarded([&] {
+        if (!S || !host_out) fail(PRC_ERR_INVALID, "null argument");
+        HIPCHK(hipMemcpy(host_out, S->f, S->A->n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+const float* prc_solver_estimate_device(prc_solver* S) { return S ? S->f : nullptr; }
+
+int prc_solver_reproject(prc_solver* S, float* host_out) {
+    return guarded([&] {
+        if (!S || !host_out) fail(PRC_ERR_INVALID, "null argument");
+        const int rc = prc_projector_forward(S->A, S->f, S->est);
+        if (rc) throw Failure{rc};
+        HIPCHK(hipMemcpy(host_out, S->est, S->A->n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+int prc_solver_sums(prc_solver* S, double* estimate_sum, double* support_sum) {
+    return guarded([&] {
+        if (!S) fail(PRC_ERR_INVALID, "null argument");
+        prc_projector* A = S->A;
+        k_sum_partial<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, S->f, S->partial);
+        k_reduce_volume<<<1, 256>>>(S->item_begin, S->partial, S->sc, 3);
+        HIPCHK(hipGetLastError());
+        Scal sc;
+        HIPCHK(hipMemcpy(&sc, S->sc, sizeof(Scal), hipMemcpyDeviceToHost));
+        if (estimate_sum) *estimate_sum = sc.sum;
+        if (support_sum) *support_sum = S->support_sum;
+    });
+}
+
+int prc_batch_correct(int n, prc_projector* const* projectors, const float* const* host_g,
+                      const prc_solve_params* params, int iterations, float* const* host_out) {
+    return guarded([&] {
+        if (n <= 0 || !projectors || !host_g || !params || !host_out || iterations < 0)
+            fail(PRC_ERR_INVALID, "invalid argument");
+        // Concatenate descriptors and item lists; volumes keep their own weights and tables.
+        std::vector<Vol> vols(n);
+        std::vector<int4> all, act;
+        std::vector<int> begin(n + 1, 0);
+        std::vector<Scal> sc(n);
+        size_t total = 0;
+        int wbytes = projectors[0]->wbytes;
+        bool accel = params[0].accelerate != 0, prior = false;
+        for (int v = 0; v < n; ++v) {
+            prc_projector* A = projectors[v];
+            if (!A) fail(PRC_ERR_INVALID, "null projector");
+            if (A->wbytes != wbytes) fail(PRC_ERR_INVALID, "a batch must share one precision");
+            if ((params[v].accelerate != 0) != accel) fail(PRC_ERR_INVALID, "a batch must share the acceleration flag");
+            prior = prior || params[v].beta > 0.0f;
+            vols[v] = A->hv;
+            vols[v].vbase = total;
+            begin[v] = (int)all.size();
+            for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
+            for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
+            total += A->n;
+        }
+        begin[n] = (int)all.size();
+        std::vector<float> g(total);
+        for (int v = 0; v < n; ++v) {
+            prc_projector* A = projectors[v];
+            const size_t base = vols[v].vbase;
+            double support_sum = 0.0;
+            for (size_t i = 0; i < A->n; ++i) {
+                if (!std::isfinite(host_g[v][i])) fail(PRC_ERR_INVALID, "Non-finite activity in volume %d", v);
+                const float x = std::max(host_g[v][i], 0.0f);
+                g[base + i] = A->h_support[i] ? x : 0.0f;
+                if (A->h_support[i]) support_sum += x;
+            }
+            if (support_sum == 0.0) fail(PRC_ERR_NO_ACTIVITY, "No activity in volume %d", v);
+            sc[v] = Scal{};
+            sc[v].support_sum = support_sum;
+            sc[v].eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);
+            sc[v].lambda = 0.0f, sc[v].scale = 1.0f, sc[v].beta = params[v].beta, sc[v].gamma = params[v].gamma;
+            sc[v].accel = accel;
+        }
+        Batch B;
+        std::vector<void*> owned;
+        auto own = [&](void* p) { owned.push_back(p); return p; };
+        try {
+            Vol* dv = (Vol*)own(dalloc<Vol>(n));
+            int4* d_all = (int4*)own(dalloc<int4>(all.size()));
+            int4* d_act = (int4*)own(dalloc<int4>(std::max<size_t>(act.size(), 1)));
+            int* d_begin = (int*)own(dalloc<int>(n + 1));
+            HIPCHK(hipMemcpy(dv, vols.data(), n * sizeof(Vol), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_act, act.data(), act.size() * sizeof(int4), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+            B.dv = dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
+            B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
+            float** bufs[] = {&B.g, &B.f, &B.prev, &B.y, &B.hprev, &B.est, &B.gf};
+            for (float** b : bufs) *b = (float*)own(dalloc<float>(total));
+            HIPCHK(hipMemcpy(B.g, g.data(), total * sizeof(float), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(B.f, g.data(), total * sizeof(float), hipMemcpyHostToDevice));
+            for (float* b : {B.prev, B.y, B.hprev, B.est}) HIPCHK(hipMemset(b, 0, total * sizeof(float)));
+            B.sc = (Scal*)own(dalloc<Scal>(n));
+            HIPCHK(hipMemcpy(B.sc, sc.data(), n * sizeof(Scal), hipMemcpyHostToDevice));
+            B.partial = (double*)own(dalloc<double>(2 * all.size()));
+            HIPCHK(hipStreamCreate(&B.stream));
+            if (iterations > 0) {
+                batch_initialise(B);
+                // Capture the whole iteration loop once and replay it as a graph.
+                hipGraph_t graph;
+                hipGraphExec_t exec;
+                HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+                Batch C = B;
+                for (int it = 0; it < iterations; ++it) {
+                    batch_iteration(C);
+                    batch_rotate(C);
+                }
+                HIPCHK(hipStreamEndCapture(B.stream, &graph));
+                HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+                HIPCHK(hipGraphLaunch(exec, B.stream));
+                HIPCHK(hipStreamSynchronize(B.stream));
+                HIPCHK(hipGraphExecDestroy(exec));
+                HIPCHK(hipGraphDestroy(graph));
+                B.f = C.f;
+            }
+            for (int v = 0; v < n; ++v)
+                HIPCHK(hipMemcpy(host_out[v], B.f + vols[v].vbase, projectors[v]->n * sizeof(float),
+                                 hipMemcpyDeviceToHost));
+        } catch (...) {
+            if (B.stream) (void)hipStreamDestroy(B.stream);
+            for (void* p : owned) (void)hipFree(p);
+            throw;
+        }
+        (void)hipStreamDestroy(B.stream);
+        for (void* p : owned) (void)hipFree(p);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""GPU parity of the HIP projector and solver against the CPU oracle (FP32 reference
+semantics and FP64), on the reference's anisotropic test phantom and small variants."""
+import numpy as np
+import pytest
+
+from conftest import announce, rel_l2
+from prc_mlem_amd import phantoms
+
+pytestmark = pytest.mark.gpu
+SPACING = (1.1, 1.6, 2.3)
+ISOTOPES = ["11C", "13N", "15O", "18F", "64Cu", "68Ga", "82Rb", "124I"]
+
+
+@pytest.fixture(scope="module")
+def phantom(oracle_mod):
+    rho, x, y, spacing = phantoms.test_phantom()
+    rho_p = np.maximum(rho, np.float32(1e-3))
+    return rho_p, x, y, rho_p >= np.float32(0.05)
+
+
+@pytest.fixture(scope="module")
+def pair(prc, oracle_mod, phantom):
+    """GPU projector and the FP32 oracle built on the GPU's own survival table."""
+    rho, x, y, sup = phantom
+    announce("build GPU projector 16x14x12 68Ga rays=32")
+    A = prc.Projector(rho, SPACING, "68Ga", 32)
+    table, _ = A.table
+    O = oracle_mod.Projector(rho, SPACING, "68Ga", 32, table=table)
+    return A, O
+
+
+def test_device_range_table_matches_host_table(prc, oracle_mod, pair):
+    # Device expf/powf vs host libm; the reference bound vs FP64 is 2e-5 (tests/test_rdp.py:24).
+    A, _ = pair
+    dev, scale = A.table
+    host, hscale = oracle_mod.range_table("68Ga")
+    assert scale == pytest.approx(float(hscale), rel=0, abs=0)
+    assert np.max(np.abs(dev - host)) <= 2e-7
+    r = np.arange(4097, dtype=np.float64) / scale
+    exact = np.array([oracle_mod.survival_f64("68Ga", v) for v in r])
+    assert np.max(np.abs(dev - exact)) <= 2e-6
+
+
+def test_normalisation_matches_oracle(pair, phantom):
+    A, O = pair
+    _, _, _, sup = phantom
+    W = A.W
+    w_ref = np.zeros(W.size)
+    w_ref[O.active] = O.W
+    np.testing.assert_array_equal(W[~sup], 0)
+    # Same table, same march arithmetic and summation order: agreement to float rounding.
+    np.testing.assert_allclose(W.ravel(), w_ref, rtol=2e-6, atol=0)
+
+
+def test_forward_adjoint_match_oracle(pair, phantom):
+    A, O = pair
+    _, x, y, sup = phantom
+    announce("forward/adjoint 16x14x12")
+    ax, aty = A.forward(x), A.adjoint(y)
+    assert rel_l2(ax, O.forward(x)) <= 1e-6
+    assert rel_l2(aty, O.adjoint(y)) <= 1e-6
+
+
+def test_identity_conservation_and_support(pair, phantom):
+    # tests/test_solver.py:180-190, with the north-star bound 1e-6.
+    A, _ = pair
+    _, x, y, sup = phantom
+    ax, aty = A.forward(x), A.adjoint(y)
+    lhs = np.sum(ax.astype(np.float64) * y)
+    rhs = np.sum(x.astype(np.float64) * aty)
+    assert abs(lhs / rhs - 1) <= 1e-6
+    assert abs(ax.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) <= 1e-6
+    np.testing.assert_array_equal(ax[~sup], 0)
+    at_one = A.adjoint(np.ones_like(x))
+    np.testing.assert_allclose(at_one[sup], 1, rtol=3e-6, atol=3e-6)
+    np.testing.assert_array_equal(at_one[~sup], 0)
+
+
+@pytest.mark.parametrize("iso", ISOTOPES)
+def test_table_vs_exact_range_all_isotopes(prc, phantom, iso):
+    # tests/test_solver.py:201-214: table lookup vs analytic survival per step.
+    rho, x, y, sup = phantom
+    announce(f"isotope {iso}: table and exact projectors")
+    A = prc.Projector(rho, SPACING, iso, 32)
+    E = prc.Projector(rho, SPACING, iso, 32, exact_range=True)
+    ax, ex = A.forward(x), E.forward(x)
+    aty, ety = A.adjoint(y), E.adjoint(y)
+    np.testing.assert_allclose(ax, ex, rtol=1e-5, atol=2e-7)
+    np.testing.assert_allclose(aty, ety, rtol=1e-5, atol=2e-7)
+    lhs = np.sum(ax.astype(np.float64) * y)
+    rhs = np.sum(x.astype(np.float64) * aty)
+    assert abs(lhs / rhs - 1) <= 1e-6
+    assert abs(ax.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) <= 1e-6
+
+
+@pytest.mark.parametrize("beta,accel", [(0.0, False), (0.1, True), (0.2, False), (0.0, True)])
+def test_solver_per_iteration_matches_oracle(prc, pair, phantom, beta, accel):
+    # FP32 GPU vs the FP32 oracle within 1e-5 relative L2 per iteration.
+    A, O = pair
+    _, x, _, sup = phantom
+    announce(f"solver beta={beta} accel={accel}")
+    _, hist = O.solve(x, 6, beta, 20.0, accel, history=True)
+    s = prc.Solver(A, x, beta, 20.0, accel)
+    for it, ref in enumerate(hist, 1):
+        s.iterate(1)
+        est = s.estimate()
+        assert rel_l2(est, ref) <= 1e-5, f"iteration {it}"
+        np.testing.assert_array_equal(est[~sup], 0)
+    tot, support_sum = s.sums()
+    assert abs(tot / support_sum - 1) <= 1e-6
+
+
+def test_mixed_precision_against_fp64(prc, oracle_mod, phantom):
+    rho, x, y, sup = phantom
+    announce("mixed precision projector")
+    M = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    O64 = oracle_mod.Projector(rho, SPACING, "68Ga", 32, prec=64)
+    ref = O64.solve(x, 6, 0.03, 20.0, True)
+    out = prc.correct(M, x, 6, 0.03, 20.0, True)
+    assert rel_l2(out, ref) <= 1e-3
+    ax, aty = M.forward(x), M.adjoint(y)
+    lhs = np.sum(ax.astype(np.float64) * y)
+    rhs = np.sum(x.astype(np.float64) * aty)
+    assert abs(lhs / rhs - 1) <= 1e-6
+    assert abs(ax.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) <= 1e-6
+
+
+def test_batch_equals_single_runs(prc, phantom):
+    rho, x, y, sup = phantom
+    announce("batched correction of three volumes")
+    A1 = prc.Projector(rho, SPACING, "68Ga", 32)
+    rho2 = np.ascontiguousarray(rho[:, ::-1, :])
+    A2 = prc.Projector(rho2, (0.9, 1.2, 1.5), "18F", 16)
+    x2 = np.ascontiguousarray(x[:, ::-1, :])
+    singles = [prc.correct(A1, x, 5, 0.1, 20.0, True), prc.correct(A2, x2, 5, 0.05, 20.0, True),
+               prc.correct(A1, y, 5, 0.0, 20.0, True)]
+    batch = prc.batch_correct([A1, A2, A1], [x, x2, y], 5, beta=[0.1, 0.05, 0.0], gamma=20.0, accelerate=True)
+    for a, b in zip(batch, singles):
+        np.testing.assert_array_equal(a, b)

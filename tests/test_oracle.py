@@ -1,0 +1,320 @@
+"""Pins of the CPU oracle (oracle/prc_oracle.cpp) against the reference's known answers,
+closed forms and independent brute-force computations. CPU only."""
+import math
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize_scalar
+
+from conftest import rel_l2
+from prc_mlem_amd import phantoms
+
+ISOTOPES = ["11C", "13N", "15O", "18F", "64Cu", "68Ga", "82Rb", "124I"]
+# Cutoffs quoted for the reference model (FP64 evaluation of prc.cu:65-91 in the survey of
+# the reference): 68Ga 8.487, 18F 2.115, 82Rb 16.16, 124I 9.66, 11C 3.772, 13N 4.979,
+# 15O 7.811, 64Cu 2.202 mm.
+QUOTED_CUTOFF = {"11C": 3.772, "13N": 4.979, "15O": 7.811, "18F": 2.115, "64Cu": 2.202, "68Ga": 8.487,
+                 "82Rb": 16.16, "124I": 9.66}
+
+
+def survival64(branches, r):
+    """Independent FP64 survival, tests/reference.py:20-26 evaluated in numpy."""
+    r = np.asarray(r, np.float64)
+    s = np.zeros_like(r)
+    for w, a, b, c, f in np.asarray(branches, np.float64):
+        x, y = b * r, c * r
+        s += w * np.exp(-((a * r) ** (1 + f) + x * x * np.sqrt(x) + y * y * y * np.sqrt(y)))
+    return s
+
+
+@pytest.mark.parametrize("iso", ISOTOPES)
+def test_model_renormalisation_and_cutoff(oracle_mod, iso):
+    br = oracle_mod.model(iso)
+    assert np.sum(br[:, 0], dtype=np.float32) == pytest.approx(1.0, abs=1e-7)
+    c = oracle_mod.range_cutoff(iso)
+    assert c == pytest.approx(QUOTED_CUTOFF[iso], rel=5e-4)  # quoted to 4 digits
+    # the cutoff holds 99.95% of the positrons (prc.cu:587)
+    assert 1 - survival64(br, c) == pytest.approx(0.9995, abs=2e-6)
+    assert survival64(br, 0.0) == pytest.approx(1.0, abs=1e-7)
+    assert oracle_mod.survival_f64(iso, 0.7) == pytest.approx(float(survival64(br, 0.7)), rel=1e-14)
+
+
+@pytest.mark.parametrize("iso", ISOTOPES)
+def test_range_table_error_below_2e5(oracle_mod, iso):
+    # tests/test_rdp.py:18-24: |S_table - S_fp64| <= 2e-5 densely and at every bin midpoint.
+    table, scale = oracle_mod.range_table(iso)
+    br = oracle_mod.model(iso)
+    cutoff = np.float32(oracle_mod.range_cutoff(iso))
+    w = np.concatenate([np.linspace(0, cutoff, 20001, dtype=np.float32),
+                        (np.arange(4096, dtype=np.float32) + np.float32(0.5)) / scale])
+    rng = np.random.default_rng(5)
+    w = np.concatenate([w, rng.uniform(0, cutoff, 4000).astype(np.float32)])
+    look = np.array([oracle_mod.survival_at(table, scale, v) for v in w])
+    assert np.max(np.abs(look - survival64(br, w))) <= 2e-5
+    np.testing.assert_allclose(table.astype(np.float64), survival64(br, np.arange(4097) / np.float64(scale)),
+                               atol=3e-7)
+    assert np.all(np.diff(table) <= 0)
+
+
+def test_ray_set_invariants(oracle_mod):
+    for n in (2, 32, 128, 2048):
+        dirs = np.array([oracle_mod.ray_f64(k, n)[1] for k in range(n)])
+        starts = np.array([oracle_mod.ray_f64(k, n)[0] for k in range(n)])
+        np.testing.assert_allclose(np.linalg.norm(dirs, axis=1), 1, atol=1e-12)
+        np.testing.assert_allclose(dirs[0::2] + dirs[1::2], 0, atol=1e-15)  # mirrored pairs
+        np.testing.assert_allclose(starts[0::2] + starts[1::2], 0, atol=1e-15)
+        assert np.all(np.abs(starts) <= 0.5)
+        # Fibonacci hemisphere: z spans (0, 1] uniformly, so the mean |z| is 1/2.
+        assert np.mean(np.abs(dirs[:, 2])) == pytest.approx(0.5, abs=1.0 / n + 1e-12)
+        d32 = np.array([oracle_mod.ray_f32(k, n)[1] for k in range(n)])
+        np.testing.assert_allclose(d32, dirs, atol=3e-7)
+
+
+def brute_force_walk(start, d, spacing, v, shape, n_max=400):
+    """Voxels crossed by the ray v + start + t d (t in mm) and the path length in each, by
+    exact FP64 plane intersections (no incremental DDA)."""
+    nz, ny, nx = shape
+    sp = np.asarray(spacing, np.float64)
+    p0 = (np.asarray(v, np.float64) + np.asarray(start, np.float64)) * sp  # mm, voxel i spans [i-0.5, i+0.5]
+    d = np.asarray(d, np.float64)
+    ts = [0.0]
+    for a in range(3):
+        if d[a] == 0:
+            continue
+        for m in range(-n_max, n_max):
+            t = ((m + 0.5) * sp[a] - p0[a]) / d[a]
+            if t > 0:
+                ts.append(t)
+    ts = np.unique(np.array(ts))
+    out = []
+    for t0, t1 in zip(ts[:-1], ts[1:]):
+        mid = p0 + 0.5 * (t0 + t1) * d
+        ijk = np.floor(mid / sp + 0.5).astype(int)
+        if not (0 <= ijk[0] < nx and 0 <= ijk[1] < ny and 0 <= ijk[2] < nz):
+            break
+        out.append((tuple(ijk), t1 - t0, np.min(np.diff(ts[max(0, len(out) - 1):len(out) + 3]))))
+    return out
+
+
+def test_dda_matches_brute_force_walk(oracle_mod):
+    """The oracle's voxel walk (prc.cu:134-165) against exact plane intersections: same voxels,
+    deposits S(w_in) - S(w_out) with FP64 path lengths, stop at the cutoff."""
+    rng = np.random.default_rng(11)
+    shape, spacing = (9, 10, 11), (1.3, 0.9, 1.7)
+    rho = rng.uniform(0.3, 1.5, shape).astype(np.float32)
+    br = oracle_mod.model("68Ga")
+    cutoff = oracle_mod.range_cutoff("68Ga")
+    checked = 0
+    for k in range(64):
+        start, d = oracle_mod.ray_f32(k, 64)
+        v = (5, 4, 4)
+        walk = brute_force_walk(start.astype(np.float64), d.astype(np.float64), spacing, v, shape)
+        if min(w[2] for w in walk[:12]) < 1e-4:
+            continue  # a near tie between two face crossings: FP32 may order them either way
+        steps, idx, wts = oracle_mod.trace_f32(rho, spacing, v, start, d, "68Ga", near=False, support_rho=0.05)
+        # replay in FP64 along the brute-force path, with the oracle's far-field block rule
+        w_in, dep_t, dep_w = 0.0, [], []
+        for (i, j, kk), seg, _ in walk:
+            w_out = min(w_in + float(rho[kk, j, i]) * seg, cutoff)
+            in_block = max(abs(i - v[0]), abs(j - v[1]), abs(kk - v[2])) <= 2
+            p = survival64(br, w_in) - survival64(br, w_out)
+            if not in_block and p > 0:
+                dep_t.append((kk * shape[1] + j) * shape[2] + i)
+                dep_w.append(float(p))
+            w_in = w_out
+            if w_out >= cutoff:
+                break
+        assert list(idx) == dep_t
+        np.testing.assert_allclose(wts, dep_w, rtol=2e-4, atol=3e-6)
+        checked += 1
+    assert checked >= 48
+
+
+def test_near_trace_telescopes(oracle_mod):
+    # In uniform tissue a near ray deposits 1 - S(w at the block exit) in total.
+    rho = np.ones((7, 7, 7), np.float32)
+    table = oracle_mod.range_table("18F")
+    for k in range(0, 2048, 97):
+        start, d = oracle_mod.ray_f32(k, 2048)
+        steps, idx, wts = oracle_mod.trace_f32(rho, (0.5, 0.5, 0.5), (3, 3, 3), start, d, "18F", near=True,
+                                               table=table)
+        assert 1 <= steps <= 7
+        assert np.all(wts > 0)
+        assert wts.sum() <= 1.0 + 1e-6
+
+
+def water_kernel(branches, voxel_mm, half_width, cutoff_mm=30.0, n_dirs=1000, n_shells=600):
+    """Voxel-averaged homogeneous water kernel by radial shells x Fibonacci directions with
+    trilinear deposition (the quadrature of tests/reference.py:53-82), independent of any
+    ray traversal. Returns an array [z, y, x] of offsets +-half_width."""
+    edges = cutoff_mm * np.linspace(0, 1, n_shells + 1) ** 2
+    mass = -np.diff(survival64(branches, edges))
+    radius = 0.5 * (edges[1:] + edges[:-1])
+    k = np.arange(n_dirs)
+    z = 1 - (2 * k + 1) / n_dirs
+    phi = 2 * np.pi * k / ((1 + 5 ** 0.5) / 2)
+    dirs = np.stack([np.sqrt(1 - z * z) * np.cos(phi), np.sqrt(1 - z * z) * np.sin(phi), z], 1)
+    n = 2 * half_width + 1
+    out = np.zeros(n ** 3)
+    for s0 in range(0, n_shells, 100):
+        p = (radius[s0:s0 + 100, None, None] * dirs / np.asarray(voxel_mm)).reshape(-1, 3)
+        m = np.repeat(mass[s0:s0 + 100], n_dirs) / n_dirs
+        base = np.floor(p).astype(int)
+        frac = p - base
+        for corner in range(8):
+            o = np.array([(corner >> axis) & 1 for axis in range(3)])
+            idx = base + o + half_width
+            ok = np.all((idx >= 0) & (idx < n), axis=1)
+            weight = m * np.prod(np.where(o, frac, 1 - frac), axis=1)
+            out += np.bincount(((idx[ok, 2] * n) + idx[ok, 1]) * n + idx[ok, 0], weight[ok], n ** 3)
+    return out.reshape(n, n, n)
+
+
+@pytest.mark.parametrize("size", [4.0, 2.0])
+def test_oracle_water_kernel_within_one_percent_of_blur(oracle_mod, size):
+    # tests/test_solver.py:216-243 on the oracle: point source in water, 68Ga, 128 rays.
+    half = int(np.ceil(10 / size)) + 1
+    n, c = 2 * half + 5, half + 2
+    delta = np.zeros((n, n, n), np.float32)
+    delta[c, c, c] = 1
+    P = oracle_mod.Projector(np.ones((n, n, n), np.float32), (size,) * 3, "68Ga", 128)
+    out = P.forward(delta)[c - half:c + half + 1, c - half:c + half + 1, c - half:c + half + 1].astype(np.float64)
+    exact = water_kernel(oracle_mod.model("68Ga").astype(np.float64), (size,) * 3, half)
+    blur = exact.copy()
+    blur[half, half, half] -= 1
+    assert np.linalg.norm(out - exact) / np.linalg.norm(blur) < 0.01
+    for axis in np.indices(out.shape) - half:  # mirrored rays: no sub-voxel shift
+        assert abs((out * axis).sum()) < 1e-6
+
+
+@pytest.fixture(scope="module")
+def small(oracle_mod):
+    rho, x, y, spacing = phantoms.test_phantom()
+    rho = np.maximum(rho, np.float32(1e-3))
+    return rho, x, y, spacing, rho >= np.float32(0.05)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_oracle_projector_invariants(oracle_mod, small, prec):
+    # tests/test_solver.py:180-190
+    rho, x, y, spacing, sup = small
+    P = oracle_mod.Projector(rho, spacing, "68Ga", 32, prec=prec)
+    ax, aty = P.forward(x), P.adjoint(y)
+    assert abs(np.sum(ax * y.astype(np.float64)) / np.sum(x.astype(np.float64) * aty) - 1) < 1e-7
+    assert abs(ax.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) < 1e-7
+    np.testing.assert_array_equal(ax[~sup], 0)
+    one = P.adjoint(np.ones_like(x))
+    np.testing.assert_allclose(one[sup], 1, rtol=1e-6)
+    np.testing.assert_array_equal(one[~sup], 0)
+    W = P.W
+    assert np.all(W > 0) and np.all(W <= 2 * 32 + 1e-3)  # each ray deposits at most 1
+
+
+def test_oracle_fp32_close_to_fp64(oracle_mod, small):
+    rho, x, _, spacing, _ = small
+    P32 = oracle_mod.Projector(rho, spacing, "68Ga", 32)
+    P64 = oracle_mod.Projector(rho, spacing, "68Ga", 32, prec=64)
+    assert rel_l2(P32.W, P64.W) < 1e-5
+    assert rel_l2(P32.forward(x), P64.forward(x)) < 1e-5
+
+
+def test_column_normalisation_exact(oracle_mod, small):
+    rho, _, _, spacing, _ = small
+    P = oracle_mod.Projector(rho, spacing, "68Ga", 32, prec=64)
+    for a in (0, 17, 301):
+        idx, w = P.column(a)
+        assert w.sum() == pytest.approx(P.W[a], rel=1e-12)
+
+
+def rdp_grid(fj, fk, back, spacing=(1.0, 1.0, 1.0)):
+    f = np.array([fj, fk], np.float64)
+    b = np.array([back, 0.0], np.float64)
+    return f, b, np.ones(2, np.uint8), spacing
+
+
+def test_rdp_update_known_answers(oracle_mod):
+    # tests/test_rdp.py:85-90: surrogate_step(old=0, em=0, G=-3, C=4, beta=1) = 0.5. A voxel
+    # with f_j = 0 and one neighbour f_k = 4 (gamma = eps = 0, unit distance weight) has
+    # G = -3 and C = 2 * max(2/4, 8/4 ... ) = 4, so its update is 0.5.
+    L = oracle_mod.lib()
+    f, b, sup, sp = rdp_grid(0.0, 4.0, 1.0)
+    assert L.orc_rdp_update_f64(2, 1, 1, *sp, f, b, sup, 0, 1.0, 0.0, 0.0) == pytest.approx(0.5, abs=1e-15)
+    # em = 0 with B >= 0 gives 0 (the second known answer's branch).
+    f, b, sup, sp = rdp_grid(1.0, 1.0, 0.0)
+    assert L.orc_rdp_update_f64(2, 1, 1, *sp, f, b, sup, 0, 0.01, 0.0, 0.0) == 0.0
+    # beta = 0 is Richardson-Lucy: f * back.
+    f, b, sup, sp = rdp_grid(2.0, 5.0, 1.5)
+    assert L.orc_rdp_update_f64(2, 1, 1, *sp, f, b, sup, 0, 0.0, 20.0, 0.1) == 3.0
+    # off support the update is 0
+    assert L.orc_rdp_update_f64(2, 1, 1, *sp, f, b, np.array([0, 1], np.uint8), 0, 0.2, 20.0, 0.1) == 0.0
+
+
+def penalty(x, y, gamma, eps):
+    return (x - y) ** 2 / (x + y + gamma * abs(x - y) + eps)
+
+
+def test_rdp_update_matches_finite_differences_and_minimisation(oracle_mod):
+    """One neighbour at unit weight: the update must be the minimiser of the surrogate
+    -em log x + x + beta (G dx + C dx^2 / 2) with G the FD gradient of the RDP penalty and
+    C = 2 max(half_quad, FD curvature) (prc.cu:279-309)."""
+    L = oracle_mod.lib()
+    rng = np.random.default_rng(42)
+    for _ in range(60):
+        fj, fk = np.exp(rng.uniform(-2, 2, 2))
+        back = np.exp(rng.uniform(-1, 1))
+        beta, gamma, eps = np.exp(rng.uniform(-3, 0)), rng.choice([0.0, 2.0, 20.0]), rng.choice([0.0, 0.17])
+        f, b, sup, sp = rdp_grid(fj, fk, back)
+        got = L.orc_rdp_update_f64(2, 1, 1, *sp, f, b, sup, 0, beta, gamma, eps)
+        h = 1e-6 * max(fj, 1e-3)
+        G = (penalty(fj + h, fk, gamma, eps) - penalty(fj - h, fk, gamma, eps)) / (2 * h)
+
+        def grad(x):
+            return (penalty(x + h, fk, gamma, eps) - penalty(x - h, fk, gamma, eps)) / (2 * h)
+
+        hess = (grad(fj + 1e-4 * fj) - grad(fj - 1e-4 * fj)) / (2e-4 * fj)
+        D = fj + fk + gamma * abs(fj - fk) + eps
+        half_quad = (2 * (fj + fk + eps) + gamma * abs(fj - fk)) / D ** 2
+        C = 2 * max(half_quad, hess)
+        em = fj * back
+
+        def objective(x):
+            dx = x - fj
+            return -em * math.log(x) + x + beta * (G * dx + C * dx * dx / 2)
+
+        opt = minimize_scalar(objective, bounds=(1e-12, 10 * (em + fj) + 10), method="bounded",
+                              options={"xatol": 1e-12})
+        assert got == pytest.approx(opt.x, rel=2e-4, abs=1e-7)
+
+
+def test_oracle_solver_invariants(oracle_mod, small):
+    # tests/test_solver.py:245-312 on the oracle.
+    rho, x, _, spacing, sup = small
+    P = oracle_mod.Projector(rho, spacing, "68Ga", 32)
+    src = x.copy()
+    src[5, 4, 5] = 0
+    np.testing.assert_array_equal(P.solve(src, 0, 0.2), src)  # zero iterations: bit-exact
+    for accel in (True, False):
+        out = P.solve(x, 25, 0.0, 20.0, accel)
+        assert out.min() >= 0
+        np.testing.assert_array_equal(out[~sup], 0)
+        assert abs(out.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) < 5e-6
+    a = P.solve(x, 4, 0.1)
+    b = P.solve(x * 1000, 4, 0.1) / 1000
+    np.testing.assert_allclose(b, a, rtol=5e-5, atol=5e-6)
+    for it in (1, 2):  # Biggs-Andrews warm-up equals plain steps
+        np.testing.assert_allclose(P.solve(x, it, 0.2, 20.0, True), P.solve(x, it, 0.2, 20.0, False),
+                                   rtol=3e-5, atol=3e-6)
+    sparse = x.copy()
+    sparse[np.random.default_rng(21).random(x.shape) < 0.5] = 0
+    out = P.solve(sparse, 5, 0.1)
+    assert np.mean(out[(sparse == 0) & sup] > 0) > 0.95
+
+
+def test_oracle_vrs_counts(oracle_mod, small):
+    rho, x, _, spacing, _ = small
+    P = oracle_mod.Projector(rho, spacing, "68Ga", 32)
+    near, far = P.vrs
+    n_a = len(P.active)
+    assert 1 * 2048 * n_a <= near <= 7 * 2048 * n_a
+    P.adjoint(x)
+    assert P.vrs[1] > 32 * n_a
