@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Corrected volumes/s on the 128x128x159 68Ga thorax workload: mixed + batched vs plain FP32.
+
+One step = the full correction of one batch: the normalisation pass (every system-matrix
+weight marched and stored) followed by 6 accelerated MAP-EM iterations (beta 0.03, gamma
+20, 128 far rays), for each volume of the batch. Inputs (density and measured counts) are
+resident in HBM before the timed region; nothing is copied over PCIe inside it.
+
+Prints one JSON line. `value` is the mixed + batched throughput; the plain FP32 path (one
+volume per launch, no graph), the per-kernel timings (HIP events, a profiled step outside
+the timed region), the roofline of the dominant kernel and a CPU baseline (the oracle on a
+bounded slab of the same workload, extrapolated by its voxel-ray-step rate) sit beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import prc_mlem_amd as prc  # noqa: E402
+from prc_mlem_amd import phantoms  # noqa: E402
+
+SHAPE = (159, 128, 128)
+SPACING = (0.776, 0.776, 0.795)
+ISOTOPE, RAYS, BETA, GAMMA, ITERS, COUNTS = "68Ga", 128, 0.03, 20.0, 6, 5e7
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def workload(seed):
+    rho, truth = phantoms.thorax_phantom(SHAPE, SPACING)
+    return rho, truth
+
+
+def timed(fn, steps, warmup, sync):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(rho, data, slices, seconds_budget):
+    """The FP32 oracle (reference semantics, OpenMP) on a Z-slab of the workload: build +
+    6 iterations. Returns the measured slab time, its voxel-ray-step rate and threads."""
+    from oracle import oracle
+    z0 = SHAPE[0] // 2 - slices // 2
+    r = np.ascontiguousarray(rho[z0:z0 + slices])
+    g = np.ascontiguousarray(data[z0:z0 + slices])
+    r = np.maximum(r, np.float32(1e-3))
+    t0 = time.perf_counter()
+    P = oracle.Projector(r, SPACING, ISOTOPE, RAYS)
+    P.solve(np.where(r >= np.float32(0.05), np.maximum(g, 0), 0).astype(np.float32), ITERS, BETA, GAMMA, True)
+    t = time.perf_counter() - t0
+    near, far = P.vrs
+    return t, near + far, oracle.threads()
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4, help="volumes per batched step")
+    ap.add_argument("--only", choices=["both", "mixed", "fp32"], default="both")
+    ap.add_argument("--cpu-slices", type=int, default=8, help="Z slices of the CPU-baseline slab (0 = skip)")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE {world}) is not supported; the "
+              "batched volumes would be sharded across GPUs by whole volumes with no communication.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    prc.lib()
+    log("phantom + blurred data")
+    rho, truth = workload(0)
+    A_blur = prc.Projector(rho, SPACING, ISOTOPE, RAYS)
+    blurred = A_blur.forward(truth)
+    A_blur.close()
+    datas = [phantoms.poisson_counts(blurred, COUNTS, seed=2000 + v) for v in range(max(args.batch, 1))]
+    n = rho.size
+    result = {}
+
+    # Plain FP32: one volume per launch, direct launches.
+    fp32 = None
+    if args.only in ("both", "fp32"):
+        log("plain fp32: build")
+        A = prc.Projector(rho, SPACING, ISOTOPE, RAYS, precision="fp32")
+        S = prc.Solver(A, datas[0], BETA, GAMMA, True)
+
+        def step_fp32():
+            A.normalise()
+            S.reset()
+            S.iterate(ITERS)
+
+        t = timed(step_fp32, args.steps, args.warmup, S.sync)
+        st = A.stats
+        fp32 = dict(vols_per_s=args.steps / t, ms_per_volume=1e3 * t / args.steps, weight_bytes=st["weight_bytes"],
+                    weight_entries=st["weight_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
+                    n_active=st["n_active"])
+        est32 = S.estimate()
+        log(f"plain fp32: {fp32['vols_per_s']:.3f} vols/s, weights {st['weight_bytes'] / 2**30:.2f} GiB")
+        if not args.no_profile:
+            prc.profile_enable(True)
+            step_fp32()
+            S.sync()
+            fp32["kernels"] = prc.profile_report()
+            prc.profile_enable(False)
+        S.close()
+        A.close()
+
+    # Mixed + batched: FP16 weights, every stage one launch over the batch, graph-captured loop.
+    mixed = None
+    if args.only in ("both", "mixed"):
+        log(f"mixed+batched x{args.batch}: build")
+        As = [prc.Projector(rho, SPACING, ISOTOPE, RAYS, precision="mixed") for _ in range(args.batch)]
+        B = prc.Batch(As, datas[:args.batch], BETA, GAMMA, True)
+
+        def step_mixed():
+            for a in As:
+                a.normalise()
+            B.run(ITERS)
+
+        t = timed(step_mixed, args.steps, args.warmup, B.sync)
+        st = As[0].stats
+        mixed = dict(vols_per_s=args.batch * args.steps / t, ms_per_step=1e3 * t / args.steps,
+                     weight_bytes_per_volume=st["weight_bytes"], weight_entries=st["weight_entries"],
+                     vrs_near=st["vrs_near"], vrs_far=st["vrs_far"])
+        est_mixed = B.estimate(0)
+        log(f"mixed+batched: {mixed['vols_per_s']:.3f} vols/s")
+        if not args.no_profile:
+            prc.profile_enable(True)
+            step_mixed()
+            B.sync()
+            mixed["kernels"] = prc.profile_report()
+            prc.profile_enable(False)
+        if fp32 is not None:
+            d = est_mixed.astype(np.float64) - est32
+            result["nrmse_mixed_vs_fp32"] = float(np.linalg.norm(d) / np.linalg.norm(est32))
+            result["count_error_mixed_vs_fp32"] = float(est_mixed.sum(dtype=np.float64) /
+                                                        est32.sum(dtype=np.float64) - 1)
+        B.close()
+        for a in As:
+            a.close()
+
+    head = mixed if mixed is not None else fp32
+    value = head["vols_per_s"]
+    k = head.get("kernels", {})
+
+    # Roofline of the dominant kernel of the headline step (by total event time). The stored-
+    # weight projector passes are HBM streams: algorithmic bytes per launch = stored weights of
+    # every volume in the launch + the voxel vectors they read and write.
+    roofline = None
+    if k:
+        name = max(k, key=lambda kk: k[kk]["total_ms"])
+        vols = args.batch if mixed is not None else 1
+        wbytes = (mixed["weight_bytes_per_volume"] if mixed is not None else fp32["weight_bytes"]) * vols
+        n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
+        per_launch = {"forward": wbytes + vols * (4 * n + 4 * n),
+                      "adjoint": wbytes + vols * (4 * n + 8 * n_act + n)}
+        if name in per_launch:
+            mean_s = k[name]["mean_ms"] / 1e3
+            achieved = per_launch[name] / mean_s / 1e9
+            roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_launch": per_launch[name], "mean_ms": k[name]["mean_ms"]}
+        else:
+            roofline = {"kernel": name, "bound": "valu", "achieved": None, "peak": None, "unit": None,
+                        "frac": None, "traffic": None, "mean_ms": k[name]["mean_ms"]}
+
+    cpu = None
+    if args.cpu_slices > 0 and fp32 is not None:
+        log(f"cpu baseline: oracle on a {args.cpu_slices}-slice slab")
+        t_slab, vrs_slab, cores = cpu_baseline(rho, datas[0], args.cpu_slices, 30)
+        rate = vrs_slab / t_slab
+        vrs_full = fp32["vrs_near"] + (1 + 2 * ITERS) * fp32["vrs_far"]
+        cpu = {"value": round(rate / vrs_full, 6), "unit": "corrected vols/s", "cores": cores, "kind": "port",
+               "sample": f"FP32 oracle: normalisation + {ITERS} iterations on a {args.cpu_slices}-slice Z slab "
+                         f"({t_slab:.1f} s, {vrs_slab:.3g} voxel-ray steps); vols/s extrapolated from its "
+                         f"{rate:.3g} steps/s to the full volume's {vrs_full:.3g} steps"}
+
+    out = {
+        "metric": "corrected vols/s + projector GB/s, 128x128x159 68Ga: mixed+batched vs FP32",
+        "value": round(value, 4), "unit": "corrected vols/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head.get("ms_per_step", head.get("ms_per_volume")), 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f16 weights, f32 arithmetic, f64 reductions" if mixed is not None else "f32",
+        "data": "synthetic thorax phantom (seeded), forward-blurred and Poisson-noised to 5e7 counts",
+        "config": {"workload": "68Ga thorax 128x128x159 @ 0.776x0.776x0.795 mm, 128 far rays, beta 0.03, "
+                               "gamma 20, Biggs-Andrews on, 6 iterations incl. normalisation",
+                   "batch": args.batch if mixed is not None else 1, "form": "mixed+batched" if mixed else "fp32"},
+        "fp32_vols_per_s": round(fp32["vols_per_s"], 4) if fp32 else None,
+        "speedup_mixed_batched_vs_fp32": round(mixed["vols_per_s"] / fp32["vols_per_s"], 3) if (mixed and fp32) else None,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    out.update(result)
+    if fp32 is not None:
+        out["fp32"] = fp32
+    if mixed is not None:
+        out["mixed"] = mixed
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

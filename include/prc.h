@@ -45,6 +45,7 @@ enum {
 
 typedef struct prc_projector prc_projector;
 typedef struct prc_solver prc_solver;
+typedef struct prc_batch prc_batch;
 
 /* Geometry and model of one volume: the `Grid` of prc.cu:94-100 plus Options
  * (prc.cu:452-459) that shape the system matrix. */
@@ -101,6 +102,10 @@ float prc_range_cutoff(int isotope, float fraction);
 int prc_projector_create(const prc_geometry* geometry, const float* host_rho, const float* host_table,
                          prc_projector** out);
 void prc_projector_destroy(prc_projector* projector);
+/* Run the normalisation pass again on the resident density (same result; reuses the weight
+ * buffer). Stream-ordered: returns after one device synchronisation (the buffer size) with
+ * the march still queued. For callers that time or repeat the build without re-uploading. */
+int prc_projector_normalise(prc_projector* projector);
 
 /* dev_out = A dev_in, overwriting every voxel (Projector::forward, prc.cu:430-434). */
 int prc_projector_forward(prc_projector* projector, const float* dev_in, float* dev_out);
@@ -126,6 +131,8 @@ const float* prc_projector_norm_device(prc_projector* projector);
 int prc_solver_create(prc_projector* projector, const float* host_g, const prc_solve_params* params,
                       prc_solver** out);
 void prc_solver_destroy(prc_solver* solver);
+/* Restart from the preprocessed input already on the device (as after create). */
+int prc_solver_reset(prc_solver* solver);
 /* Run n more iterations on the device (no host synchronisation inside). */
 int prc_solver_iterate(prc_solver* solver, int n);
 /* Wait for queued work. */
@@ -139,12 +146,29 @@ int prc_solver_reproject(prc_solver* solver, float* host_out);
 /* Sum of the current estimate in FP64 and the supported input counts. */
 int prc_solver_sums(prc_solver* solver, double* estimate_sum, double* support_sum);
 
-/* Batched correction of n independent volumes: every stage of every iteration runs as
- * one launch over all volumes' work, and the iteration loop is captured once as a HIP
- * graph and replayed. No counterpart in the reference (one volume per run); results
- * equal n separate prc_solver runs. host_out[v] receives volume v's estimate. */
+/* Batched correction of n independent volumes (no counterpart in the reference, which runs
+ * one volume per process). The volumes' data are concatenated on the device; every stage of
+ * an iteration is one launch over all volumes' work, and the iteration loop is captured once
+ * as a HIP graph (per iteration count) and replayed. Results equal separate solver runs.
+ * Volumes must share the precision and the acceleration flag; beta and gamma may differ. */
+int prc_batch_create(int n, prc_projector* const* projectors, const float* const* host_g,
+                     const prc_solve_params* params, prc_batch** out);
+/* Restart every volume from its data and run `iterations` (asynchronous). The batch reads
+ * the projectors' current weights, so a projector may be normalised again between runs. */
+int prc_batch_run(prc_batch* batch, int iterations);
+int prc_batch_sync(prc_batch* batch);
+/* Copy volume v's estimate after the last run (synchronises). */
+int prc_batch_estimate(prc_batch* batch, int v, float* host_out);
+void prc_batch_destroy(prc_batch* batch);
+/* create + run + estimates + destroy in one call. */
 int prc_batch_correct(int n, prc_projector* const* projectors, const float* const* host_g,
                       const prc_solve_params* params, int iterations, float* const* host_out);
+
+/* Kernel timing: while enabled, every launch of the library is bracketed by HIP events on its
+ * stream (graph replay is bypassed). The report is JSON {name: {count, total_ms, mean_ms,
+ * min_ms, max_ms}} and clears the records. No counterpart in the reference. */
+int prc_profile_enable(int on);
+int prc_profile_report(char* json, size_t capacity);
 
 #ifdef __cplusplus
 }

@@ -63,6 +63,7 @@ SIGNATURES = {
     "prc_range_cutoff": (C.c_float, [C.c_int, C.c_float]),
     "prc_projector_create": (C.c_int, [C.POINTER(Geometry), _F32P, C.c_void_p, C.POINTER(C.c_void_p)]),
     "prc_projector_destroy": (None, [C.c_void_p]),
+    "prc_projector_normalise": (C.c_int, [C.c_void_p]),
     "prc_projector_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_forward_host": (C.c_int, [C.c_void_p, _F32P, _F32P]),
@@ -73,14 +74,23 @@ SIGNATURES = {
     "prc_projector_norm_device": (C.c_void_p, [C.c_void_p]),
     "prc_solver_create": (C.c_int, [C.c_void_p, _F32P, C.POINTER(SolveParams), C.POINTER(C.c_void_p)]),
     "prc_solver_destroy": (None, [C.c_void_p]),
+    "prc_solver_reset": (C.c_int, [C.c_void_p]),
     "prc_solver_iterate": (C.c_int, [C.c_void_p, C.c_int]),
     "prc_solver_sync": (C.c_int, [C.c_void_p]),
     "prc_solver_estimate": (C.c_int, [C.c_void_p, _F32P]),
     "prc_solver_estimate_device": (C.c_void_p, [C.c_void_p]),
     "prc_solver_reproject": (C.c_int, [C.c_void_p, _F32P]),
     "prc_solver_sums": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "prc_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                   C.POINTER(SolveParams), C.POINTER(C.c_void_p)]),
+    "prc_batch_run": (C.c_int, [C.c_void_p, C.c_int]),
+    "prc_batch_sync": (C.c_int, [C.c_void_p]),
+    "prc_batch_estimate": (C.c_int, [C.c_void_p, C.c_int, _F32P]),
+    "prc_batch_destroy": (None, [C.c_void_p]),
     "prc_batch_correct": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                     C.POINTER(SolveParams), C.c_int, C.POINTER(C.c_void_p)]),
+    "prc_profile_enable": (C.c_int, [C.c_int]),
+    "prc_profile_report": (C.c_int, [C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -185,6 +195,10 @@ class Projector:
     def adjoint_device(self, y_ptr, out_ptr):
         _check(lib().prc_projector_adjoint(self.handle, C.c_void_p(y_ptr), C.c_void_p(out_ptr)))
 
+    def normalise(self):
+        """Rebuild the stored weights from the resident density (asynchronous)."""
+        _check(lib().prc_projector_normalise(self.handle))
+
     @property
     def W(self):
         """Per-voxel normalisation W (0 off support)."""
@@ -224,6 +238,10 @@ class Solver:
 
     __del__ = close
 
+    def reset(self):
+        _check(lib().prc_solver_reset(self.handle))
+        return self
+
     def iterate(self, n=1):
         _check(lib().prc_solver_iterate(self.handle, int(n)))
         return self
@@ -257,22 +275,65 @@ def correct(projector, g, iterations=20, beta=0.2, gamma=20.0, accelerate=True):
         s.close()
 
 
-def batch_correct(projectors, data, iterations, beta=0.2, gamma=20.0, accelerate=True):
-    """Correct several volumes with one launch per stage and a graph-captured loop.
+class Batch:
+    """Several volumes corrected together: one launch per stage over all volumes and a
+    graph-captured iteration loop. beta/gamma may be scalars or per-volume sequences."""
 
-    beta/gamma may be scalars or one value per volume; all volumes share `accelerate`
-    and the projectors' precision."""
-    n = len(projectors)
-    betas = np.broadcast_to(np.asarray(beta, np.float32), (n,))
-    gammas = np.broadcast_to(np.asarray(gamma, np.float32), (n,))
-    params = (SolveParams * n)(*[SolveParams(float(b), float(g), int(bool(accelerate))) for b, g in zip(betas, gammas)])
-    ins = [_volume(d, p.shape) for d, p in zip(data, projectors)]
-    outs = [np.zeros(p.shape, np.float32) for p in projectors]
-    hs = (C.c_void_p * n)(*[p.handle.value for p in projectors])
-    ip = (C.c_void_p * n)(*[a.ctypes.data for a in ins])
-    op = (C.c_void_p * n)(*[a.ctypes.data for a in outs])
-    _check(lib().prc_batch_correct(n, hs, ip, params, int(iterations), op))
-    return outs
+    def __init__(self, projectors, data, beta=0.2, gamma=20.0, accelerate=True):
+        n = len(projectors)
+        self.projectors = list(projectors)
+        betas = np.broadcast_to(np.asarray(beta, np.float32), (n,))
+        gammas = np.broadcast_to(np.asarray(gamma, np.float32), (n,))
+        self.params = (SolveParams * n)(*[SolveParams(float(b), float(g), int(bool(accelerate)))
+                                          for b, g in zip(betas, gammas)])
+        ins = [_volume(d, p.shape) for d, p in zip(data, projectors)]
+        hs = (C.c_void_p * n)(*[p.handle.value for p in projectors])
+        ip = (C.c_void_p * n)(*[a.ctypes.data for a in ins])
+        h = C.c_void_p()
+        _check(lib().prc_batch_create(n, hs, ip, self.params, C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().prc_batch_destroy(self.handle)
+            self.handle = None
+
+    __del__ = close
+
+    def run(self, iterations):
+        _check(lib().prc_batch_run(self.handle, int(iterations)))
+        return self
+
+    def sync(self):
+        _check(lib().prc_batch_sync(self.handle))
+
+    def estimate(self, v):
+        out = np.zeros(self.projectors[v].shape, np.float32)
+        _check(lib().prc_batch_estimate(self.handle, int(v), out))
+        return out
+
+
+def batch_correct(projectors, data, iterations, beta=0.2, gamma=20.0, accelerate=True):
+    """One batched correction; returns the estimates."""
+    b = Batch(projectors, data, beta, gamma, accelerate)
+    try:
+        b.run(iterations)
+        return [b.estimate(v) for v in range(len(projectors))]
+    finally:
+        b.close()
+
+
+def profile_enable(on=True):
+    """Time every kernel launch with HIP events (graph replay is bypassed while on)."""
+    _check(lib().prc_profile_enable(int(bool(on))))
+
+
+def profile_report():
+    """Per-kernel {count, total_ms, mean_ms, min_ms, max_ms} since the last report."""
+    import json
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().prc_profile_report(buf, len(buf)))
+    return json.loads(buf.value.decode())
 
 
 def run_cli(*args, env=None, timeout=600):

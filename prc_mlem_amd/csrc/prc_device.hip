@@ -517,6 +517,72 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     }
 }
 
+// Plane offsets of the row-plane layout, computed on the device after the counting pass:
+// near block = 125 steps for every row holding active voxels, rows without any hold nothing.
+__global__ void k_row_totals(Vol V, long long* row_tot) {
+    const int rows = V.ny * V.nz, R1 = V.n_rays + 1;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const int len = gload(V.row_len + row);
+    long long off = 0;
+    for (int p = 0; p < R1; ++p) {
+        RowInfo* e = V.rinfo + (size_t)row * R1 + p;
+        const int cnt = len == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
+        e->cnt = cnt;
+        e->off = off;
+        off += (long long)cnt * len;
+    }
+    row_tot[row] = off;
+}
+
+// Exclusive prefix sum of the row totals by one block in a fixed order; *total = grand total.
+__global__ __launch_bounds__(1024) void k_scan_rows(const long long* __restrict__ row_tot,
+                                                      long long* __restrict__ row_base, int rows,
+                                                      long long* __restrict__ total) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int chunk = (rows + nt - 1) / nt;
+    const int b = min(rows, t * chunk), e = min(rows, b + chunk);
+    long long s = 0;
+    for (int i = b; i < e; ++i) s += row_tot[i];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < nt; o <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
+        const long long v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long base = t ? part[t - 1] : 0;
+    for (int i = b; i < e; ++i) {
+        row_base[i] = base;
+        base += row_tot[i];
+    }
+    if (t == nt - 1) *total = part[nt - 1];
+}
+
+__global__ void k_row_offsets(Vol V, const long long* __restrict__ row_base) {
+    const int R1 = V.n_rays + 1;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)V.ny * V.nz * R1) return;
+    V.rinfo[id].off += row_base[id / R1];
+}
+
+// kglob[p] = kfirst[p] + max over rows of cnt(row, p): the forward's search bound per ray.
+__global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
+    __shared__ int sm[256];
+    const int p = blockIdx.x, R1 = V.n_rays + 1, rows = V.ny * V.nz;
+    int m = 0;
+    for (int row = threadIdx.x; row < rows; row += blockDim.x) m = max(m, V.rinfo[(size_t)row * R1 + p].cnt);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) kglob[p] = sm[0] > 0 ? gload(V.kfirst + p) + sm[0] : 0;
+}
+
 // Forward bound per (row, ray): the last step at which some source row reaches this row.
 __global__ void k_forward_bounds(Vol V, const int* kglob) {
     const int R1 = V.n_rays + 1;
@@ -891,6 +957,33 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// ---------------------------------------------------------------------------------------
+// Optional per-launch timing with HIP events (prc_profile_enable). Graph replay is not used
+// while it is on, so every kernel launch is bracketed on its own stream.
+struct ProfRecord {
+    const char* name;
+    hipEvent_t begin, end;
+};
+bool g_prof_on = false;
+std::vector<ProfRecord> g_prof;
+
+struct ProfScope {
+    ProfRecord rec{};
+    hipStream_t stream;
+    bool on;
+    ProfScope(const char* name, hipStream_t s) : stream(s), on(g_prof_on) {
+        if (!on) return;
+        rec.name = name;
+        HIPCHK(hipEventCreate(&rec.begin));
+        HIPCHK(hipEventCreate(&rec.end));
+        HIPCHK(hipEventRecord(rec.begin, stream));
+    }
+    ~ProfScope() {
+        if (!on) return;
+        if (hipEventRecord(rec.end, stream) == hipSuccess) g_prof.push_back(rec);
+    }
+};
+
 }  // namespace
 
 // One volume's projector.
@@ -907,11 +1000,16 @@ struct prc_projector {
     int* row_len = nullptr;
     RowInfo* rinfo = nullptr;
     void* wbuf = nullptr;
+    long long wcap = 0;  // weight buffer capacity (entries)
     FarStep* fstep = nullptr;
     int4* pstep = nullptr;
     int* kfirst = nullptr;
     NearStep* nstep = nullptr;
     int* nlen = nullptr;
+    int* kglob = nullptr;
+    long long* row_tot = nullptr;
+    long long* row_base = nullptr;
+    long long* total = nullptr;
     int4* items_act = nullptr;  // active row segments (march, adjoint)
     int4* items_all = nullptr;  // every row segment (forward, streaming)
     int n_act = 0, n_all = 0;
@@ -919,7 +1017,6 @@ struct prc_projector {
     std::vector<unsigned char> h_support;
     unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
     float* scratch_in = nullptr;
-    float* scratch_out = nullptr;
     prc_stats stats{};
     size_t n = 0;
     int wbytes = 4;
@@ -930,13 +1027,13 @@ struct prc_solver {
     prc_solve_params p{};
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
+    float* base[5] = {};  // f, prev, y, hprev, est as allocated (the loop rotates them)
     Scal* sc = nullptr;
+    Scal sc0{};
     double* partial = nullptr;
     int* item_begin = nullptr;
     double support_sum = 0.0;
-    long long n_active = 0;
     bool initialised = false;
-    int iterations_done = 0;
 };
 
 namespace {
@@ -958,7 +1055,8 @@ void build_items(prc_projector* A) {
             }
         if (lo >= 0) {
             xa[row] = lo, len[row] = hi - lo + 1;
-            for (int x0 = lo; x0 <= hi; x0 += kWave) A->h_items_act.push_back(make_int4(0, row, x0, std::min(kWave, hi + 1 - x0)));
+            for (int x0 = lo; x0 <= hi; x0 += kWave)
+                A->h_items_act.push_back(make_int4(0, row, x0, std::min(kWave, hi + 1 - x0)));
         }
         for (int x0 = 0; x0 < nx; x0 += kWave) A->h_items_all.push_back(make_int4(0, row, x0, std::min(kWave, nx - x0)));
     }
@@ -984,9 +1082,11 @@ void launch_far(prc_projector* A, int pass) {
     const bool exact = A->hv.exact;
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
     if (pass == 1) {
+        ProfScope ps("far_count", 0);
         if (exact) k_far<1, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
         else k_far<1, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
     } else {
+        ProfScope ps("far_fill", 0);
         if (exact) k_far<2, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
         else k_far<2, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
     }
@@ -996,14 +1096,16 @@ void launch_far(prc_projector* A, int pass) {
 template <class WT>
 void launch_near(prc_projector* A) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
+    ProfScope ps("near", 0);
     if (A->hv.exact) k_near<true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
     else k_near<false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
     HIPCHK(hipGetLastError());
 }
 
-void build_projector(prc_projector* A, const float* host_rho, const float* host_table) {
+// Density upload, support, work items, range table and ray tables (everything that does not
+// depend on the march).
+void setup_projector(prc_projector* A, const float* host_rho, const float* host_table) {
     const prc_geometry& g = A->g;
-    const double t0 = now_s();
     const size_t n = (size_t)g.nx * g.ny * g.nz;
     A->n = n;
     A->model = renormalised(g.isotope);
@@ -1037,9 +1139,7 @@ void build_projector(prc_projector* A, const float* host_rho, const float* host_
     A->support = dalloc<unsigned char>(n);
     HIPCHK(hipMemcpy(A->support, A->h_support.data(), n, hipMemcpyHostToDevice));
     A->W = dalloc<float>(n);
-    HIPCHK(hipMemset(A->W, 0, n * sizeof(float)));
     A->counters = dalloc<unsigned long long>(4);
-    HIPCHK(hipMemset(A->counters, 0, 4 * sizeof(unsigned long long)));
 
     // Range table on the device (prc.cu:625-633) unless the caller supplies one.
     const float scale = (float)kBins / cutoff;
@@ -1060,6 +1160,7 @@ void build_projector(prc_projector* A, const float* host_rho, const float* host_
     A->fstep = dalloc<FarStep>((size_t)R * L);
     A->pstep = dalloc<int4>((size_t)R1 * LP);
     A->kfirst = dalloc<int>(R1);
+    A->kglob = dalloc<int>(R1);
     A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
     A->nlen = dalloc<int>(kNearRays);
     HIPCHK(hipMemset(A->nstep, 0, (size_t)kNearRays * kNearMax * sizeof(NearStep)));
@@ -1082,51 +1183,53 @@ void build_projector(prc_projector* A, const float* host_rho, const float* host_
     V.row_xa = A->row_xa, V.row_len = A->row_len;
     const size_t rows = (size_t)g.ny * g.nz;
     A->rinfo = dalloc<RowInfo>(rows * R1);
-    HIPCHK(hipMemset(A->rinfo, 0, rows * R1 * sizeof(RowInfo)));
+    A->row_tot = dalloc<long long>(rows);
+    A->row_base = dalloc<long long>(rows);
+    A->total = dalloc<long long>(1);
     V.rinfo = A->rinfo;
     V.wbuf = nullptr;
     A->dv = dalloc<Vol>(1);
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
+}
 
-    // Pass 1: count the stored steps of every (row, far ray).
+// The normalisation pass (Projector::Projector, prc.cu:405-422): count, lay out, then march
+// again writing every weight and W. One host synchronisation (the buffer size); the weight
+// buffer is reused when it is large enough.
+void normalise_projector(prc_projector* A) {
+    Vol& V = A->hv;
+    const size_t rows = (size_t)V.ny * V.nz;
+    const int R1 = V.n_rays + 1;
+    HIPCHK(hipMemsetAsync(A->rinfo, 0, rows * R1 * sizeof(RowInfo), 0));
+    HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), 0));
+    HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), 0));
     if (A->wbytes == 2) launch_far<__half>(A, 1);
     else launch_far<float>(A, 1);
-
-    // Offsets of the row planes (host prefix sum), then the weight buffer.
-    std::vector<RowInfo> ri(rows * R1);
-    HIPCHK(hipMemcpy(ri.data(), A->rinfo, rows * R1 * sizeof(RowInfo), hipMemcpyDeviceToHost));
-    std::vector<int> h_kfirst(R1);
-    HIPCHK(hipMemcpy(h_kfirst.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<int> len(rows);
-    HIPCHK(hipMemcpy(len.data(), A->row_len, rows * sizeof(int), hipMemcpyDeviceToHost));
-    std::vector<int> kglob(R1, 0);
-    long long off = 0;
-    for (size_t row = 0; row < rows; ++row) {
-        for (int p = 0; p < R1; ++p) {
-            RowInfo& e = ri[row * R1 + p];
-            if (len[row] == 0) {
-                e.cnt = 0;
-            } else if (p == 0) {
-                e.cnt = kNQ;
-            }
-            e.off = off;
-            off += (long long)e.cnt * len[row];
-            kglob[p] = std::max(kglob[p], h_kfirst[p] + e.cnt);
-        }
+    {
+        ProfScope ps("layout", 0);
+        k_row_totals<<<blocks_for((long long)rows, 256), 256>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)rows, A->total);
+        k_row_offsets<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->row_base);
+        k_kglob<<<R1, 256>>>(V, A->kglob);
+        HIPCHK(hipGetLastError());
     }
-    A->stats.weight_entries = off;
-    A->stats.weight_bytes = off * A->wbytes;
-    HIPCHK(hipMemcpy(A->rinfo, ri.data(), rows * R1 * sizeof(RowInfo), hipMemcpyHostToDevice));
-    A->wbuf = (void*)dalloc<unsigned char>((size_t)std::max<long long>(off, 1) * A->wbytes);
+    long long total = 0;
+    HIPCHK(hipMemcpy(&total, A->total, sizeof(total), hipMemcpyDeviceToHost));
+    if (total > A->wcap || !A->wbuf) {
+        if (A->wbuf) (void)hipFree(A->wbuf);
+        A->wbuf = nullptr;
+        A->wcap = 0;
+        A->wbuf = (void*)dalloc<unsigned char>((size_t)std::max<long long>(total, 1) * A->wbytes);
+        A->wcap = std::max<long long>(total, 1);
+    }
+    A->stats.weight_entries = total;
+    A->stats.weight_bytes = total * A->wbytes;
     V.wbuf = A->wbuf;
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
-
-    int* d_kglob = dalloc<int>(R1);
-    HIPCHK(hipMemcpy(d_kglob, kglob.data(), R1 * sizeof(int), hipMemcpyHostToDevice));
-    k_forward_bounds<<<blocks_for((long long)rows * R1, 256), 256>>>(V, d_kglob);
-    HIPCHK(hipGetLastError());
-
-    // Pass 2: near block, then far rays, writing every weight and W.
+    {
+        ProfScope ps("forward_bounds", 0);
+        k_forward_bounds<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->kglob);
+        HIPCHK(hipGetLastError());
+    }
     if (A->wbytes == 2) {
         launch_near<__half>(A);
         launch_far<__half>(A, 2);
@@ -1134,37 +1237,41 @@ void build_projector(prc_projector* A, const float* host_rho, const float* host_
         launch_near<float>(A);
         launch_far<float>(A, 2);
     }
-    HIPCHK(hipDeviceSynchronize());
-    dfree(d_kglob);
+}
+
+void read_counters(prc_projector* A) {
     unsigned long long c[4];
     HIPCHK(hipMemcpy(c, A->counters, sizeof(c), hipMemcpyDeviceToHost));
     A->stats.vrs_near = (long long)c[0];
     A->stats.vrs_far = (long long)c[1];
     A->stats.deposits = (long long)c[2];
-    A->stats.norm_seconds = now_s() - t0;
 }
 
 void release(prc_projector* A) {
     dfree(A->dv), dfree(A->rho_p), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
-    dfree(A->scratch_out);
+    dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total);
     if (A->wbuf) (void)hipFree(A->wbuf);
     A->wbuf = nullptr;
 }
 
 // Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
-void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out) {
+void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out,
+                 hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
-    if (wbytes == 2) k_forward<__half><<<nb, 256>>>(dv, items, n_items, g, out);
-    else k_forward<float><<<nb, 256>>>(dv, items, n_items, g, out);
+    ProfScope ps("forward", s);
+    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
     HIPCHK(hipGetLastError());
 }
 
-void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out) {
+void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out,
+                 hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
-    if (wbytes == 2) k_adjoint<__half><<<nb, 256>>>(dv, items, n_items, in, out);
-    else k_adjoint<float><<<nb, 256>>>(dv, items, n_items, in, out);
+    ProfScope ps("adjoint", s);
+    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
     HIPCHK(hipGetLastError());
 }
 
@@ -1183,54 +1290,65 @@ void check_geometry(const prc_geometry* g) {
 
 // ---------------------------------------------------------------------------------------
 // Solver (prc.cu:659-710), all volumes of a batch at once. Buffers are concatenated voxel
-// vectors; S (host) and sc (device) hold the per-volume scalars.
+// vectors; sc (device) holds the per-volume scalars.
 struct Batch {
     const Vol* dv = nullptr;
     const int4* items_all = nullptr;
     const int4* items_act = nullptr;
     int n_all = 0, n_act = 0, n_vols = 1, wbytes = 4;
     const int* item_begin = nullptr;  // per-volume ranges of items_all
-    float *g, *f, *prev, *y, *hprev, *est, *gf;
-    Scal* sc;
-    double* partial;
+    float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
+          *gf = nullptr;
+    Scal* sc = nullptr;
+    double* partial = nullptr;
     bool accel = false, any_prior = false;
     hipStream_t stream = 0;
 };
 
+// prc.cu:666-672: floor on support, then normalise the counts to the supported input sum.
 void batch_initialise(const Batch& B) {
     const unsigned nb = blocks_for(B.n_all, kItemsPerBlock);
+    ProfScope ps("initialise", B.stream);
     k_initialize<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.partial);
     k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 0);
     k_finish<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, nullptr, nullptr, nullptr, 1);
     HIPCHK(hipGetLastError());
 }
 
-// One iteration; the caller rotates the buffers with batch_rotate afterwards.
+// One iteration (prc.cu:676-700); the caller rotates the buffers with batch_rotate afterwards.
 void batch_iteration(const Batch& B) {
     const unsigned nb = blocks_for(B.n_all, kItemsPerBlock);
-    const unsigned na = blocks_for(B.n_act, kItemsPerBlock);
     const float* start = B.f;
-    if (B.accel) {
-        k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf);
-        start = B.y;
-    } else {
-        k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
+    {
+        ProfScope ps("extrapolate", B.stream);
+        if (B.accel) {
+            k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf);
+            start = B.y;
+        } else {
+            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
+        }
     }
-    if (B.wbytes == 2) k_forward<__half><<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.gf, B.est);
-    else k_forward<float><<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.gf, B.est);
-    k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
-    if (B.wbytes == 2) k_adjoint<__half><<<na, 256, 0, B.stream>>>(B.dv, B.items_act, B.n_act, B.est, B.prev);
-    else k_adjoint<float><<<na, 256, 0, B.stream>>>(B.dv, B.items_act, B.n_act, B.est, B.prev);
-    k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
-    if (B.any_prior) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 1);
-    if (B.any_prior || B.accel)
+    run_forward(B.dv, B.items_all, B.n_all, B.wbytes, B.gf, B.est, B.stream);
+    {
+        ProfScope ps("ratio", B.stream);
+        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
+    }
+    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.stream);
+    {
+        ProfScope ps("update", B.stream);
+        k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
+        if (B.any_prior) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 1);
+    }
+    if (B.any_prior || B.accel) {
+        ProfScope ps("finish", B.stream);
         k_finish<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.est, B.accel ? B.y : nullptr,
                                            B.hprev, B.accel ? B.partial : nullptr, B.any_prior ? 1 : 0);
-    if (B.accel) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 2);
+        if (B.accel) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 2);
+    }
     HIPCHK(hipGetLastError());
 }
 
-void batch_rotate(Batch& B) {
+void batch_rotate(Batch& B) {  // prc.cu:697-700
     if (B.accel) {
         std::swap(B.y, B.hprev);
         std::swap(B.prev, B.f);
@@ -1238,12 +1356,130 @@ void batch_rotate(Batch& B) {
     std::swap(B.f, B.est);
 }
 
+// Host preprocessing of measured data (prc.cu:598-613): clamp, zero off support, FP64 sum.
+double preprocess_data(const prc_projector* A, const float* host_g, float* out) {
+    double support_sum = 0.0;
+    for (size_t i = 0; i < A->n; ++i) {
+        if (!std::isfinite(host_g[i])) fail(PRC_ERR_INVALID, "Non-finite density or activity at voxel %zu", i);
+        const float v = std::max(host_g[i], 0.0f);
+        out[i] = A->h_support[i] ? v : 0.0f;
+        if (A->h_support[i]) support_sum += v;
+    }
+    if (support_sum == 0.0)
+        fail(PRC_ERR_NO_ACTIVITY, "No activity in voxels with density >= %g: nothing to do", (double)A->g.support_rho);
+    return support_sum;
+}
+
+Scal initial_scalars(const prc_projector* A, double support_sum, const prc_solve_params& p) {
+    Scal sc{};
+    sc.support_sum = support_sum;
+    sc.eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);  // prc.cu:668
+    sc.lambda = 0.0f, sc.scale = 1.0f, sc.beta = p.beta, sc.gamma = p.gamma;
+    sc.accel = p.accelerate ? 1 : 0;
+    return sc;
+}
+
+void check_params(const prc_solve_params* p) {
+    if (!(p->beta >= 0) || !(p->gamma >= 0) || !std::isfinite(p->beta) || !std::isfinite(p->gamma))
+        fail(PRC_ERR_INVALID, "Invalid geometry or solver parameters");
+}
+
+}  // namespace
+
+// Several volumes corrected together: concatenated vectors, one launch per stage, and the
+// iteration loop captured as a HIP graph (cached per iteration count).
+struct prc_batch {
+    int n = 0;
+    std::vector<prc_projector*> A;
+    std::vector<Vol> vols;
+    std::vector<Scal> sc0;
+    size_t total = 0;
+    Batch B;
+    float* base[5] = {};  // f, prev, y, hprev, est as allocated
+    std::vector<void*> owned;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int exec_iterations = -1;
+    float* final_f = nullptr;
+    Vol* dv = nullptr;
+};
+
+namespace {
+
+void batch_release(prc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->B.stream) (void)hipStreamDestroy(b->B.stream);
+    for (void* p : b->owned) (void)hipFree(p);
+    b->owned.clear();
+}
+
+// Restore the starting state (estimate = preprocessed data, zero history, initial scalars) and
+// refresh the descriptors, which change when a projector is normalised again.
+void batch_reset(prc_batch* b) {
+    Batch& B = b->B;
+    for (int v = 0; v < b->n; ++v) {
+        const size_t vb = b->vols[v].vbase;
+        b->vols[v] = b->A[v]->hv;
+        b->vols[v].vbase = vb;
+    }
+    HIPCHK(hipMemcpyAsync(b->dv, b->vols.data(), b->n * sizeof(Vol), hipMemcpyHostToDevice, B.stream));
+    B.f = b->base[0], B.prev = b->base[1], B.y = b->base[2], B.hprev = b->base[3], B.est = b->base[4];
+    HIPCHK(hipMemcpyAsync(B.f, B.g, b->total * sizeof(float), hipMemcpyDeviceToDevice, B.stream));
+    for (int i = 1; i < 5; ++i) HIPCHK(hipMemsetAsync(b->base[i], 0, b->total * sizeof(float), B.stream));
+    HIPCHK(hipMemcpyAsync(B.sc, b->sc0.data(), b->n * sizeof(Scal), hipMemcpyHostToDevice, B.stream));
+}
+
+void batch_run(prc_batch* b, int iterations) {
+    batch_reset(b);
+    Batch& B = b->B;
+    b->final_f = B.f;
+    if (iterations == 0) return;
+    batch_initialise(B);
+    if (g_prof_on) {  // direct launches so that each kernel can be timed
+        Batch C = B;
+        for (int it = 0; it < iterations; ++it) {
+            batch_iteration(C);
+            batch_rotate(C);
+        }
+        b->final_f = C.f;
+        return;
+    }
+    if (b->exec_iterations != iterations) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        if (b->graph) (void)hipGraphDestroy(b->graph);
+        b->exec = nullptr, b->graph = nullptr, b->exec_iterations = -1;
+        HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
+        Batch C = B;
+        try {
+            for (int it = 0; it < iterations; ++it) {
+                batch_iteration(C);
+                batch_rotate(C);
+            }
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(B.stream, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(B.stream, &b->graph));
+        HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->exec_iterations = iterations;
+        b->final_f = C.f;
+    } else {
+        Batch C = B;
+        for (int it = 0; it < iterations; ++it) batch_rotate(C);
+        b->final_f = C.f;
+    }
+    HIPCHK(hipGraphLaunch(b->exec, B.stream));
+}
+
 }  // namespace
 
 // =======================================================================================
 extern "C" {
 
-const char* prc_version(void) { return "prc-mi355x 0.1 (gfx950)"; }
+const char* prc_version(void) { return "prc-mi355x 0.2 (gfx950)"; }
 const char* prc_last_error(void) { return g_error.c_str(); }
 int prc_isotope_count(void) { return 8; }
 const char* prc_isotope_name(int index) { return index >= 0 && index < 8 ? kIsotopes[index].name : nullptr; }
@@ -1270,13 +1506,24 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
         prc_projector* A = new prc_projector;
         A->g = *geometry;
         try {
-            build_projector(A, host_rho, host_table);
+            const double t0 = now_s();
+            setup_projector(A, host_rho, host_table);
+            normalise_projector(A);
+            HIPCHK(hipDeviceSynchronize());
+            A->stats.norm_seconds = now_s() - t0;
         } catch (...) {
             release(A);
             delete A;
             throw;
         }
         *out = A;
+    });
+}
+
+int prc_projector_normalise(prc_projector* A) {
+    return guarded([&] {
+        if (!A) fail(PRC_ERR_INVALID, "null argument");
+        normalise_projector(A);
     });
 }
 
@@ -1290,16 +1537,20 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
     return guarded([&] {
         if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
         if (!A->scratch_in) A->scratch_in = dalloc<float>(A->n);
-        k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, A->scratch_in);
-        HIPCHK(hipGetLastError());
-        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, A->scratch_in, dev_out);
+        {
+            ProfScope ps("extrapolate", 0);
+            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in,
+                                                                    A->scratch_in);
+            HIPCHK(hipGetLastError());
+        }
+        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, A->scratch_in, dev_out, 0);
     });
 }
 
 int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out) {
     return guarded([&] {
         if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, dev_in, dev_out);
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, dev_in, dev_out, 0);
     });
 }
 
@@ -1351,6 +1602,7 @@ int prc_projector_table(prc_projector* A, float* host_table, float* scale) {
 int prc_projector_stats(prc_projector* A, prc_stats* stats) {
     return guarded([&] {
         if (!A || !stats) fail(PRC_ERR_INVALID, "null argument");
+        read_counters(A);
         *stats = A->stats;
     });
 }
@@ -1359,59 +1611,50 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
     return guarded([&] {
         if (!A || !host_g || !params || !out) fail(PRC_ERR_INVALID, "null argument");
         *out = nullptr;
-        if (!(params->beta >= 0) || !(params->gamma >= 0) || !std::isfinite(params->beta) ||
-            !std::isfinite(params->gamma))
-            fail(PRC_ERR_INVALID, "Invalid geometry or solver parameters");
+        check_params(params);
         const size_t n = A->n;
         std::vector<float> g(n);
-        double support_sum = 0.0;
-        for (size_t i = 0; i < n; ++i) {  // prc.cu:598-613
-            if (!std::isfinite(host_g[i])) fail(PRC_ERR_INVALID, "Non-finite density or activity at voxel %zu", i);
-            const float v = std::max(host_g[i], 0.0f);
-            g[i] = A->h_support[i] ? v : 0.0f;
-            if (A->h_support[i]) support_sum += v;
-        }
-        if (support_sum == 0.0) fail(PRC_ERR_NO_ACTIVITY, "No activity in voxels with density >= %g: nothing to do",
-                                     (double)A->g.support_rho);
+        const double support_sum = preprocess_data(A, host_g, g.data());
         prc_solver* S = new prc_solver;
         S->A = A;
         S->p = *params;
         S->support_sum = support_sum;
-        S->n_active = A->stats.n_active;
+        S->sc0 = initial_scalars(A, support_sum, *params);
         try {
-            S->g = dalloc<float>(n), S->f = dalloc<float>(n), S->prev = dalloc<float>(n), S->est = dalloc<float>(n);
-            S->gf = dalloc<float>(n);
-            if (params->accelerate) S->y = dalloc<float>(n), S->hprev = dalloc<float>(n);
+            S->g = dalloc<float>(n), S->gf = dalloc<float>(n);
+            for (int i = 0; i < 5; ++i) S->base[i] = dalloc<float>(n);
             HIPCHK(hipMemcpy(S->g, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(S->f, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemset(S->prev, 0, n * sizeof(float)));
-            HIPCHK(hipMemset(S->est, 0, n * sizeof(float)));
-            if (S->y) HIPCHK(hipMemset(S->y, 0, n * sizeof(float)));
-            if (S->hprev) HIPCHK(hipMemset(S->hprev, 0, n * sizeof(float)));
-            Scal sc{};
-            sc.support_sum = support_sum;
-            sc.eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);  // prc.cu:668
-            sc.lambda = 0.0f, sc.scale = 1.0f, sc.beta = params->beta, sc.gamma = params->gamma;
-            sc.accel = params->accelerate ? 1 : 0;
             S->sc = dalloc<Scal>(1);
-            HIPCHK(hipMemcpy(S->sc, &sc, sizeof(Scal), hipMemcpyHostToDevice));
             S->partial = dalloc<double>(2 * (size_t)A->n_all);
             int ib[2] = {0, A->n_all};
             S->item_begin = dalloc<int>(2);
             HIPCHK(hipMemcpy(S->item_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
+            const int rc = prc_solver_reset(S);
+            if (rc) throw Failure{rc};
         } catch (...) {
-            dfree(S->g), dfree(S->f), dfree(S->prev), dfree(S->est), dfree(S->gf), dfree(S->y), dfree(S->hprev);
-            dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
-            delete S;
+            prc_solver_destroy(S);
             throw;
         }
         *out = S;
     });
 }
 
+int prc_solver_reset(prc_solver* S) {
+    return guarded([&] {
+        if (!S) fail(PRC_ERR_INVALID, "null argument");
+        const size_t n = S->A->n;
+        S->f = S->base[0], S->prev = S->base[1], S->y = S->base[2], S->hprev = S->base[3], S->est = S->base[4];
+        HIPCHK(hipMemcpyAsync(S->f, S->g, n * sizeof(float), hipMemcpyDeviceToDevice, 0));
+        for (int i = 1; i < 5; ++i) HIPCHK(hipMemsetAsync(S->base[i], 0, n * sizeof(float), 0));
+        HIPCHK(hipMemcpy(S->sc, &S->sc0, sizeof(Scal), hipMemcpyHostToDevice));
+        S->initialised = false;
+    });
+}
+
 void prc_solver_destroy(prc_solver* S) {
     if (!S) return;
-    dfree(S->g), dfree(S->f), dfree(S->prev), dfree(S->est), dfree(S->gf), dfree(S->y), dfree(S->hprev);
+    dfree(S->g), dfree(S->gf);
+    for (float*& b : S->base) dfree(b);
     dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
     delete S;
 }
@@ -1441,7 +1684,6 @@ int prc_solver_iterate(prc_solver* S, int n) {
             batch_rotate(B);
         }
         S->f = B.f, S->prev = B.prev, S->y = B.y, S->hprev = B.hprev, S->est = B.est;
-        S->iterations_done += n;
     });
 }
 
@@ -1484,103 +1726,147 @@ int prc_solver_sums(prc_solver* S, double* estimate_sum, double* support_sum) {
     });
 }
 
-int prc_batch_correct(int n, prc_projector* const* projectors, const float* const* host_g,
-                      const prc_solve_params* params, int iterations, float* const* host_out) {
+int prc_batch_create(int n, prc_projector* const* projectors, const float* const* host_g,
+                     const prc_solve_params* params, prc_batch** out) {
     return guarded([&] {
-        if (n <= 0 || !projectors || !host_g || !params || !host_out || iterations < 0)
-            fail(PRC_ERR_INVALID, "invalid argument");
-        // Concatenate descriptors and item lists; volumes keep their own weights and tables.
-        std::vector<Vol> vols(n);
-        std::vector<int4> all, act;
-        std::vector<int> begin(n + 1, 0);
-        std::vector<Scal> sc(n);
-        size_t total = 0;
-        int wbytes = projectors[0]->wbytes;
-        bool accel = params[0].accelerate != 0, prior = false;
-        for (int v = 0; v < n; ++v) {
-            prc_projector* A = projectors[v];
-            if (!A) fail(PRC_ERR_INVALID, "null projector");
-            if (A->wbytes != wbytes) fail(PRC_ERR_INVALID, "a batch must share one precision");
-            if ((params[v].accelerate != 0) != accel) fail(PRC_ERR_INVALID, "a batch must share the acceleration flag");
-            prior = prior || params[v].beta > 0.0f;
-            vols[v] = A->hv;
-            vols[v].vbase = total;
-            begin[v] = (int)all.size();
-            for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
-            for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
-            total += A->n;
-        }
-        begin[n] = (int)all.size();
-        std::vector<float> g(total);
-        for (int v = 0; v < n; ++v) {
-            prc_projector* A = projectors[v];
-            const size_t base = vols[v].vbase;
-            double support_sum = 0.0;
-            for (size_t i = 0; i < A->n; ++i) {
-                if (!std::isfinite(host_g[v][i])) fail(PRC_ERR_INVALID, "Non-finite activity in volume %d", v);
-                const float x = std::max(host_g[v][i], 0.0f);
-                g[base + i] = A->h_support[i] ? x : 0.0f;
-                if (A->h_support[i]) support_sum += x;
-            }
-            if (support_sum == 0.0) fail(PRC_ERR_NO_ACTIVITY, "No activity in volume %d", v);
-            sc[v] = Scal{};
-            sc[v].support_sum = support_sum;
-            sc[v].eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);
-            sc[v].lambda = 0.0f, sc[v].scale = 1.0f, sc[v].beta = params[v].beta, sc[v].gamma = params[v].gamma;
-            sc[v].accel = accel;
-        }
-        Batch B;
-        std::vector<void*> owned;
-        auto own = [&](void* p) { owned.push_back(p); return p; };
+        if (n <= 0 || !projectors || !host_g || !params || !out) fail(PRC_ERR_INVALID, "invalid argument");
+        *out = nullptr;
+        prc_batch* b = new prc_batch;
         try {
-            Vol* dv = (Vol*)own(dalloc<Vol>(n));
+            b->n = n;
+            b->A.assign(projectors, projectors + n);
+            b->vols.resize(n);
+            b->sc0.resize(n);
+            std::vector<int4> all, act;
+            std::vector<int> begin(n + 1, 0);
+            const int wbytes = projectors[0] ? projectors[0]->wbytes : 4;
+            const bool accel = params[0].accelerate != 0;
+            bool prior = false;
+            for (int v = 0; v < n; ++v) {
+                prc_projector* A = projectors[v];
+                if (!A) fail(PRC_ERR_INVALID, "null projector");
+                if (A->wbytes != wbytes) fail(PRC_ERR_INVALID, "a batch must share one precision");
+                if ((params[v].accelerate != 0) != accel) fail(PRC_ERR_INVALID, "a batch must share the acceleration flag");
+                check_params(&params[v]);
+                prior = prior || params[v].beta > 0.0f;
+                b->vols[v] = A->hv;
+                b->vols[v].vbase = b->total;
+                begin[v] = (int)all.size();
+                for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
+                for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
+                b->total += A->n;
+            }
+            begin[n] = (int)all.size();
+            std::vector<float> g(b->total);
+            for (int v = 0; v < n; ++v) {
+                const double ss = preprocess_data(projectors[v], host_g[v], g.data() + b->vols[v].vbase);
+                b->sc0[v] = initial_scalars(projectors[v], ss, params[v]);
+            }
+            auto own = [&](void* p) {
+                b->owned.push_back(p);
+                return p;
+            };
+            Batch& B = b->B;
+            b->dv = (Vol*)own(dalloc<Vol>(n));
             int4* d_all = (int4*)own(dalloc<int4>(all.size()));
             int4* d_act = (int4*)own(dalloc<int4>(std::max<size_t>(act.size(), 1)));
             int* d_begin = (int*)own(dalloc<int>(n + 1));
-            HIPCHK(hipMemcpy(dv, vols.data(), n * sizeof(Vol), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_act, act.data(), act.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
-            B.dv = dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
+            B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
             B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
-            float** bufs[] = {&B.g, &B.f, &B.prev, &B.y, &B.hprev, &B.est, &B.gf};
-            for (float** b : bufs) *b = (float*)own(dalloc<float>(total));
-            HIPCHK(hipMemcpy(B.g, g.data(), total * sizeof(float), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(B.f, g.data(), total * sizeof(float), hipMemcpyHostToDevice));
-            for (float* b : {B.prev, B.y, B.hprev, B.est}) HIPCHK(hipMemset(b, 0, total * sizeof(float)));
+            B.g = (float*)own(dalloc<float>(b->total));
+            B.gf = (float*)own(dalloc<float>(b->total));
+            for (int i = 0; i < 5; ++i) b->base[i] = (float*)own(dalloc<float>(b->total));
+            HIPCHK(hipMemcpy(B.g, g.data(), b->total * sizeof(float), hipMemcpyHostToDevice));
             B.sc = (Scal*)own(dalloc<Scal>(n));
-            HIPCHK(hipMemcpy(B.sc, sc.data(), n * sizeof(Scal), hipMemcpyHostToDevice));
             B.partial = (double*)own(dalloc<double>(2 * all.size()));
             HIPCHK(hipStreamCreate(&B.stream));
-            if (iterations > 0) {
-                batch_initialise(B);
-                // Capture the whole iteration loop once and replay it as a graph.
-                hipGraph_t graph;
-                hipGraphExec_t exec;
-                HIPCHK(hipStreamBeginCapture(B.stream, hipStreamCaptureModeThreadLocal));
-                Batch C = B;
-                for (int it = 0; it < iterations; ++it) {
-                    batch_iteration(C);
-                    batch_rotate(C);
-                }
-                HIPCHK(hipStreamEndCapture(B.stream, &graph));
-                HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-                HIPCHK(hipGraphLaunch(exec, B.stream));
-                HIPCHK(hipStreamSynchronize(B.stream));
-                HIPCHK(hipGraphExecDestroy(exec));
-                HIPCHK(hipGraphDestroy(graph));
-                B.f = C.f;
-            }
-            for (int v = 0; v < n; ++v)
-                HIPCHK(hipMemcpy(host_out[v], B.f + vols[v].vbase, projectors[v]->n * sizeof(float),
-                                 hipMemcpyDeviceToHost));
         } catch (...) {
-            if (B.stream) (void)hipStreamDestroy(B.stream);
-            for (void* p : owned) (void)hipFree(p);
+            batch_release(b);
+            delete b;
             throw;
         }
-        (void)hipStreamDestroy(B.stream);
-        for (void* p : owned) (void)hipFree(p);
+        *out = b;
+    });
+}
+
+int prc_batch_run(prc_batch* b, int iterations) {
+    return guarded([&] {
+        if (!b || iterations < 0) fail(PRC_ERR_INVALID, "invalid argument");
+        batch_run(b, iterations);
+    });
+}
+
+int prc_batch_sync(prc_batch* b) {
+    return guarded([&] {
+        if (!b) fail(PRC_ERR_INVALID, "null argument");
+        HIPCHK(hipStreamSynchronize(b->B.stream));
+    });
+}
+
+int prc_batch_estimate(prc_batch* b, int v, float* host_out) {
+    return guarded([&] {
+        if (!b || !host_out || v < 0 || v >= b->n || !b->final_f) fail(PRC_ERR_INVALID, "invalid argument");
+        HIPCHK(hipStreamSynchronize(b->B.stream));
+        HIPCHK(hipMemcpy(host_out, b->final_f + b->vols[v].vbase, b->A[v]->n * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+void prc_batch_destroy(prc_batch* b) {
+    if (!b) return;
+    batch_release(b);
+    delete b;
+}
+
+int prc_batch_correct(int n, prc_projector* const* projectors, const float* const* host_g,
+                      const prc_solve_params* params, int iterations, float* const* host_out) {
+    prc_batch* b = nullptr;
+    int rc = prc_batch_create(n, projectors, host_g, params, &b);
+    if (rc) return rc;
+    rc = prc_batch_run(b, iterations);
+    for (int v = 0; v < n && rc == PRC_OK; ++v) rc = prc_batch_estimate(b, v, host_out[v]);
+    prc_batch_destroy(b);
+    return rc;
+}
+
+int prc_profile_enable(int on) {
+    return guarded([&] {
+        HIPCHK(hipDeviceSynchronize());
+        g_prof_on = on != 0;
+    });
+}
+
+int prc_profile_report(char* json, size_t capacity) {
+    return guarded([&] {
+        HIPCHK(hipDeviceSynchronize());
+        struct Agg {
+            long long count = 0;
+            double total = 0.0, lo = 1e300, hi = 0.0;
+        };
+        std::vector<std::pair<std::string, Agg>> agg;
+        for (const ProfRecord& r : g_prof) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, r.begin, r.end));
+            auto it = std::find_if(agg.begin(), agg.end(), [&](const auto& e) { return e.first == r.name; });
+            if (it == agg.end()) agg.emplace_back(r.name, Agg{}), it = agg.end() - 1;
+            it->second.count++, it->second.total += ms;
+            it->second.lo = std::min(it->second.lo, (double)ms), it->second.hi = std::max(it->second.hi, (double)ms);
+        }
+        for (const ProfRecord& r : g_prof) (void)hipEventDestroy(r.begin), (void)hipEventDestroy(r.end);
+        g_prof.clear();
+        std::string s = "{";
+        for (size_t i = 0; i < agg.size(); ++i) {
+            char buf[256];
+            snprintf(buf, sizeof(buf), "%s\"%s\": {\"count\": %lld, \"total_ms\": %.6f, \"mean_ms\": %.6f, \"min_ms\": %.6f, \"max_ms\": %.6f}",
+                     i ? ", " : "", agg[i].first.c_str(), agg[i].second.count, agg[i].second.total,
+                     agg[i].second.total / agg[i].second.count, agg[i].second.lo, agg[i].second.hi);
+            s += buf;
+        }
+        s += "}";
+        if (!json || capacity < s.size() + 1) fail(PRC_ERR_INVALID, "report needs %zu bytes", s.size() + 1);
+        memcpy(json, s.c_str(), s.size() + 1);
     });
 }
 

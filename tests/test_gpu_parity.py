@@ -137,3 +137,44 @@ def test_batch_equals_single_runs(prc, phantom):
     batch = prc.batch_correct([A1, A2, A1], [x, x2, y], 5, beta=[0.1, 0.05, 0.0], gamma=20.0, accelerate=True)
     for a, b in zip(batch, singles):
         np.testing.assert_array_equal(a, b)
+
+
+def test_renormalise_reset_and_rerun_reproduce(prc, phantom):
+    # The timed paths rebuild the weights and restart from resident data: results must repeat.
+    rho, x, y, sup = phantom
+    announce("normalise again / solver reset / batch re-run")
+    A = prc.Projector(rho, SPACING, "68Ga", 32)
+    W0, ax0 = A.W, A.forward(x)
+    A.normalise()
+    np.testing.assert_array_equal(A.W, W0)
+    np.testing.assert_array_equal(A.forward(x), ax0)
+    s = prc.Solver(A, x, 0.1, 20.0, True)
+    first = s.iterate(4).estimate()
+    second = s.reset().iterate(4).estimate()
+    np.testing.assert_array_equal(first, second)
+    b = prc.Batch([A, A], [x, y], beta=0.1, gamma=20.0, accelerate=True)
+    b.run(4)
+    r1 = [b.estimate(0), b.estimate(1)]
+    A.normalise()
+    b.run(4)
+    np.testing.assert_array_equal(b.estimate(0), r1[0])
+    np.testing.assert_array_equal(b.estimate(1), r1[1])
+    np.testing.assert_array_equal(r1[0], first)
+    b.run(3)  # another iteration count captures another graph
+    np.testing.assert_array_equal(b.estimate(0), s.reset().iterate(3).estimate())
+
+
+def test_profiled_run_matches_graph_run(prc, phantom):
+    rho, x, y, sup = phantom
+    A = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    b = prc.Batch([A], [x], beta=0.03, gamma=20.0, accelerate=True)
+    ref = b.run(5).estimate(0)
+    prc.profile_enable(True)
+    try:
+        out = b.run(5).estimate(0)
+        rep = prc.profile_report()
+    finally:
+        prc.profile_enable(False)
+    np.testing.assert_array_equal(out, ref)
+    assert rep["forward"]["count"] == 5 and rep["adjoint"]["count"] == 5
+    assert all(v["mean_ms"] >= 0 for v in rep.values())
