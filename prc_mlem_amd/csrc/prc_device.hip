@@ -44,6 +44,9 @@ constexpr int kNearMax = 8;              // a near ray visits at most 7 block vo
 constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
 constexpr int kWave = 64;
 constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
+constexpr int kFwdUnroll = 4;            // forward steps resolved per batch of loads
+constexpr int kAdjUnroll = 8;            // adjoint steps per batch of loads
+constexpr int kFarGroup = 4;             // far-march steps whose density loads are issued together
 
 // ---------------------------------------------------------------------------------------
 // Isotope model (prc.cu:38-57): Paneque-Yunta et al., Phys. Med. Biol. 70 (2025) 165013.
@@ -108,10 +111,12 @@ float range_cutoff_host(const Isotope& m, float fraction) {
 
 // ---------------------------------------------------------------------------------------
 // Device data layout.
-struct RowInfo {        // one per (row, ray)
+struct RowInfo {        // one per (row, ray); 32 bytes, one scalar load
     long long off;      // element offset of the row's planes for this ray in the weight buffer
     int cnt;            // stored steps
     int kt;             // forward bound: last step (exclusive) at which this row receives weight
+    int xa, len;        // the row's active x range [xa, xa + len) (len 0: no active voxel)
+    int pad0, pad1;
 };
 struct FarStep {        // march table of a far ray
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
@@ -181,10 +186,12 @@ struct Scal {
 #define GLOBAL_AS __attribute__((address_space(1)))
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
 
 template <class T>
 __device__ inline T gload(const T* p) {
-    if constexpr (sizeof(T) == 16) return __builtin_bit_cast(T, *(const GLOBAL_AS v4i*)p);
+    if constexpr (sizeof(T) == 32) return __builtin_bit_cast(T, *(const GLOBAL_AS v8i*)p);
+    else if constexpr (sizeof(T) == 16) return __builtin_bit_cast(T, *(const GLOBAL_AS v4i*)p);
     else if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, *(const GLOBAL_AS v2i*)p);
     else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, *(const GLOBAL_AS int*)p);
     else if constexpr (sizeof(T) == 2) return __builtin_bit_cast(T, *(const GLOBAL_AS unsigned short*)p);
@@ -399,26 +406,32 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     for (int r = 0; r < kNearRays; ++r) {
         const int len = gload(V.nlen + r);
         const NearStep* nsr = V.nstep + r * kNearMax;
+        // The whole ray stays inside the 5x5x5 block: load its densities up front (the +inf
+        // margin around the padded density keeps every address in bounds).
+        NearStep ns[7];
+        float rv[7];
+#pragma unroll
+        for (int s = 0; s < 7; ++s) ns[s] = s < len ? gload(nsr + s) : NearStep{0.0f, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 7; ++s) rv[s] = (s < len && act) ? gload(rho0 + ns[s].dp + lane) : INFINITY;
         Lane st;
         st.alive = act;
-        auto step = [&](int s, int& loc) {
-            const NearStep ns = gload(nsr + s);
-            loc = ns.loc;
-            return st.alive ? march_step<EXACT>(st, gload(rho0 + ns.dp + lane), ns.seg, true, V, tab, visits) : 0.0f;
+        auto step = [&](int s) {
+            return st.alive ? march_step<EXACT>(st, rv[s], ns[s].seg, true, V, tab, visits) : 0.0f;
         };
-        int loc;
-        a0 += step(0, loc);  // every near ray starts inside the source voxel
-        if (len > 1) { const float d = step(1, loc); a1[loc] += d; }
-        if (len > 2) { const float d = step(2, loc); a2[loc] += d; }
-        if (len > 3) { const float d = step(3, loc); a3[loc] += d; }
+        a0 += step(0);  // every near ray starts inside the source voxel
+        if (len > 1) { const float d = step(1); a1[ns[1].loc] += d; }
+        if (len > 2) { const float d = step(2); a2[ns[2].loc] += d; }
+        if (len > 3) { const float d = step(3); a3[ns[3].loc] += d; }
         if (len > 4) {  // both halves are updated, branch-free, to keep the registers in place
-            const float d = step(4, loc);
+            const float d = step(4);
+            const int loc = ns[4].loc;
             const bool lo = loc < 18;
             a4l[lo ? loc : 0] += lo ? d : 0.0f;
             a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
         }
-        if (len > 5) { const float d = step(5, loc); a5[loc] += d; }
-        if (len > 6) { const float d = step(6, loc); a6[loc] += d; }
+        if (len > 5) { const float d = step(5); a5[ns[5].loc] += d; }
+        if (len > 6) { const float d = step(6); a6[ns[6].loc] += d; }
     }
     auto bin = [&](int q) {
         const int sh = kShellMap.shell[q], lc = kShellMap.local[q];
@@ -481,20 +494,34 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
         st.alive = act;
         int last = -1;
         int s = 0;
-        for (; s < V.L; ++s) {
-            const FarStep f = gload(fs + s);
-            float dep = 0.0f;
-            if (st.alive) dep = march_step<EXACT>(st, gload(rho0 + f.dp + lane), f.seg, s >= kf, V, tab, visits);
-            if (PASS == 1) {
-                if (dep > 0.0f) last = s;
-            } else if (s >= kf && s - kf < cnt) {
-                const WT h = WeightIO<WT>::store_value(dep);
-                const float wv = WeightIO<WT>::to_float(h);
-                if (wv > 0.0f) wacc += wv, ++deposits;
-                if (valid) gstore(wp + (size_t)s * len + lane, h);
+        for (; s < V.L; s += kFarGroup) {
+            // Density loads of the whole group are issued before any step is evaluated; a lane
+            // alive at the group start is at most kFarGroup voxels outside the grid by its end,
+            // inside the +inf margin of the padded density.
+            FarStep f[kFarGroup];
+            float rv[kFarGroup];
+            const bool go = st.alive;
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) f[u] = s + u < V.L ? gload(fs + s + u) : FarStep{0.0f, 0};
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) rv[u] = go ? gload(rho0 + f[u].dp + lane) : INFINITY;
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {
+                const int su = s + u;
+                if (su >= V.L) break;
+                float dep = 0.0f;
+                if (st.alive) dep = march_step<EXACT>(st, rv[u], f[u].seg, su >= kf, V, tab, visits);
+                if (PASS == 1) {
+                    if (dep > 0.0f) last = su;
+                } else if (su >= kf && su - kf < cnt) {
+                    const WT h = WeightIO<WT>::store_value(dep);
+                    const float wv = WeightIO<WT>::to_float(h);
+                    if (wv > 0.0f) wacc += wv, ++deposits;
+                    if (valid) gstore(wp + (size_t)su * len + lane, h);
+                }
             }
             if (!__any(st.alive)) {
-                ++s;
+                s += kFarGroup;
                 break;
             }
         }
@@ -523,13 +550,15 @@ __global__ void k_row_totals(Vol V, long long* row_tot) {
     const int rows = V.ny * V.nz, R1 = V.n_rays + 1;
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
-    const int len = gload(V.row_len + row);
+    const int len = gload(V.row_len + row), xa = gload(V.row_xa + row);
     long long off = 0;
     for (int p = 0; p < R1; ++p) {
         RowInfo* e = V.rinfo + (size_t)row * R1 + p;
         const int cnt = len == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
         e->cnt = cnt;
         e->off = off;
+        e->xa = xa;
+        e->len = len;
         off += (long long)cnt * len;
     }
     row_tot[row] = off;
@@ -618,8 +647,11 @@ __device__ inline float forward_scale(float f, float w) {
 }
 
 // out = A g where g already holds f / W (prc.cu:239-254 as a gather). Every voxel of the
-// item is written (zeros where nothing arrives), so out needs no clearing.
-template <class WT>
+// item is written (zeros where nothing arrives), so out needs no clearing. U steps are
+// resolved on the scalar unit first without branches (source row, plane base, valid lane
+// range), then their 2U vector loads are issued together: the loads depend on geometry only.
+// Lanes outside the valid range load a clamped in-range element and are discarded by a select.
+template <class WT, int U>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ gin,
                                                   float* __restrict__ out) {
@@ -628,34 +660,63 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
     const int4 it = items[item_id];
     if (it.w <= 0) return;
     const Vol& V = vols[it.x];
+    const int nx = V.nx, ny = V.ny, nz = V.nz, R1 = V.n_rays + 1, LP = V.LP;
+    const RowInfo* rinfo = V.rinfo;
+    const int4* pstep = V.pstep;
+    const int* kfirst = V.kfirst;
+    const WT* wb = (const WT*)V.wbuf;
     const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
-    float* o = out + V.vbase + (size_t)row * V.nx + x0;
-    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
-    if (len == 0 || x0 + it.w <= xa || x0 >= xa + len) {
+    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    float* o = out + V.vbase + (size_t)row * nx + x0;
+    const RowInfo own = gload(rinfo + (size_t)row * R1);
+    if (own.len == 0 || x0 + it.w <= own.xa || x0 >= own.xa + own.len) {
         if (lane < it.w) o[lane] = 0.0f;
         return;
     }
     const float* g = gin + V.vbase;
-    const WT* wb = (const WT*)V.wbuf;
-    const int R1 = V.n_rays + 1;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
-        const int kf = gload(V.kfirst + p);
-        const int kend = gload(V.rinfo + (size_t)row * R1 + p).kt;
-        const int4* ps = V.pstep + (size_t)p * V.LP;
-        for (int k = kf; k < kend; ++k) {
-            const int4 d = gload(ps + k);
-            const int ys = y - d.y, zs = z - d.z;
-            if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
-            const int srow = zs * V.ny + ys;
-            const RowInfo ri = gload(V.rinfo + (size_t)srow * R1 + p);
-            if (k - kf >= ri.cnt) continue;
-            const int sxa = gload(V.row_xa + srow), slen = gload(V.row_len + srow);
-            const int xs = x0 - d.x + lane;
-            if ((unsigned)(xs - sxa) < (unsigned)slen) {
-                const float w = WeightIO<WT>::load(wb + ri.off + (size_t)(k - kf) * slen + (xs - sxa));
-                if (w != 0.0f) acc = fmaf(w, gload(g + (size_t)srow * V.nx + xs), acc);
+        const int kf = gload(kfirst + p);
+        const int kend = gload(rinfo + (size_t)row * R1 + p).kt;
+        const int4* ps = pstep + (size_t)p * LP;
+        for (int k = kf; k < kend; k += U) {
+            int4 d[U];
+            int srow[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) d[u] = gload(ps + min(k + u, kend - 1));
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ys = y - d[u].y, zs = z - d[u].z;
+                ok[u] = (k + u < kend) & ((unsigned)ys < (unsigned)ny) & ((unsigned)zs < (unsigned)nz);
+                srow[u] = ok[u] ? zs * ny + ys : row;
+            }
+            RowInfo ri[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) ri[u] = gload(rinfo + (size_t)srow[u] * R1 + p);
+            const WT* wptr[U];
+            const float* gptr[U];
+            int lo[U], hi[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int xs0 = x0 - d[u].x;  // source x of lane 0
+                const bool live = ok[u] & (k + u - kf < ri[u].cnt) & (ri[u].len > 0);
+                lo[u] = live ? ri[u].xa - xs0 : 0;
+                hi[u] = live ? lo[u] + ri[u].len : 0;
+                wptr[u] = live ? wb + ri[u].off + (size_t)(k + u - kf) * ri[u].len + (xs0 - ri[u].xa) : wb;
+                gptr[u] = live ? g + (size_t)srow[u] * nx + xs0 : g;
+            }
+            float w[U], gv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int lc = max(lo[u], min(lane, hi[u] - 1));  // in range; lane 0 of wb / g when empty
+                w[u] = WeightIO<WT>::load(wptr[u] + lc);
+                gv[u] = gload(gptr[u] + lc);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool m = (lane >= lo[u]) & (lane < hi[u]) & (w[u] != 0.0f);
+                acc = m ? fmaf(w[u], gv[u], acc) : acc;
             }
         }
     }
@@ -663,7 +724,9 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the planes.
-template <class WT>
+// The gather of in[] is bounded by the target's position (inside the grid), not by the weight,
+// so U weight loads and U gathers issue together; zero weights are skipped by a select.
+template <class WT, int U>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ in,
                                                   float* __restrict__ out) {
@@ -672,29 +735,50 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     const int4 it = items[item_id];
     if (it.w <= 0) return;
     const Vol& V = vols[it.x];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
-    const bool valid = lane < it.w;
-    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
-    const float* src = in + V.vbase;
+    const int nx = V.nx, ny = V.ny, nz = V.nz, R1 = V.n_rays + 1, LP = V.LP;
+    const RowInfo* rinfo = V.rinfo;
+    const int4* pstep = V.pstep;
+    const int* kfirst = V.kfirst;
     const WT* wb = (const WT*)V.wbuf;
-    const int R1 = V.n_rays + 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const int lw = min(lane, it.w - 1);  // clamped lane for the weight stream
+    const float* src = in + V.vbase;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
-        const RowInfo ri = gload(V.rinfo + (size_t)row * R1 + p);
-        const int kf = gload(V.kfirst + p);
-        const WT* wp = wb + ri.off + (x0 - xa);
-        const int4* ps = V.pstep + (size_t)p * V.LP + kf;
-        for (int j = 0; j < ri.cnt; ++j) {
-            const int4 d = gload(ps + j);
-            if (valid) {
-                const float w = WeightIO<WT>::load(wp + (size_t)j * len + lane);
-                if (w != 0.0f)
-                    acc = fmaf(w, gload(src + ((size_t)(z + d.z) * V.ny + (y + d.y)) * V.nx + x0 + d.x + lane), acc);
+        const RowInfo ri = gload(rinfo + (size_t)row * R1 + p);
+        const int kf = gload(kfirst + p);
+        const WT* wp = wb + ri.off + (x0 - ri.xa) + lw;
+        const int4* ps = pstep + (size_t)p * LP + kf;
+        for (int j = 0; j < ri.cnt; j += U) {
+            int4 d[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) d[u] = gload(ps + min(j + u, ri.cnt - 1));
+            const float* sp[U];
+            int lo[U], hi[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ty = y + d[u].y, tz = z + d[u].z;
+                const bool ok = (j + u < ri.cnt) & ((unsigned)ty < (unsigned)ny) & ((unsigned)tz < (unsigned)nz);
+                lo[u] = ok ? -(x0 + d[u].x) : 0;  // lanes whose target x is inside the grid
+                hi[u] = ok ? lo[u] + nx : 0;
+                sp[u] = ok ? src + ((size_t)tz * ny + ty) * nx + (x0 + d[u].x) : src;
+            }
+            float w[U], v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                w[u] = WeightIO<WT>::load(wp + (size_t)min(j + u, ri.cnt - 1) * ri.len);
+                v[u] = gload(sp[u] + max(lo[u], min(lane, hi[u] - 1)));
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool m = (j + u < ri.cnt) & (lane >= lo[u]) & (lane < hi[u]) & (w[u] != 0.0f);
+                acc = m ? fmaf(w[u], v[u], acc) : acc;
             }
         }
     }
-    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const size_t loc = (size_t)row * V.nx + x0 + lw;
     if (valid && gload(V.support + loc)) {
         const float wv = gload(V.W + loc);
         out[V.vbase + loc] = wv > 0.0f ? acc / wv : 0.0f;
@@ -992,7 +1076,8 @@ struct prc_projector {
     Isotope model{};
     Vol hv{};
     Vol* dv = nullptr;
-    float* rho_p = nullptr;
+    float* rho_alloc = nullptr;  // padded density with its +inf margin
+    float* rho_p = nullptr;      // first voxel of the padded volume inside rho_alloc
     float* table = nullptr;
     unsigned char* support = nullptr;
     float* W = nullptr;
@@ -1117,13 +1202,15 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     // Density: clamp (prc.cu:603), support mask (prc.cu:605), +inf halo of width one voxel.
     const int nxp = g.nx + 2, nyp = g.ny + 2, nzp = g.nz + 2;
     const size_t np = (size_t)nxp * nyp * nzp;
-    std::vector<float> rp(np, INFINITY);
+    // +inf margin of (kFarGroup + 1) padded slices on both sides for the march's prefetches.
+    const size_t margin = (size_t)(kFarGroup + 1) * nxp * nyp;
+    std::vector<float> rp(np + 2 * margin, INFINITY);
     A->h_support.assign(n, 0);
     long long n_active = 0;
     for (int k = 0; k < g.nz; ++k)
         for (int j = 0; j < g.ny; ++j) {
             const size_t src = ((size_t)k * g.ny + j) * g.nx;
-            float* dst = rp.data() + (((size_t)(k + 1) * nyp + (j + 1)) * nxp + 1);
+            float* dst = rp.data() + margin + (((size_t)(k + 1) * nyp + (j + 1)) * nxp + 1);
             for (int i = 0; i < g.nx; ++i) {
                 const float r = std::max(host_rho[src + i], kMinDensity);
                 dst[i] = r;
@@ -1133,8 +1220,9 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
             }
         }
     A->stats.n_active = n_active;
-    A->rho_p = dalloc<float>(np);
-    HIPCHK(hipMemcpy(A->rho_p, rp.data(), np * sizeof(float), hipMemcpyHostToDevice));
+    A->rho_alloc = dalloc<float>(np + 2 * margin);
+    HIPCHK(hipMemcpy(A->rho_alloc, rp.data(), (np + 2 * margin) * sizeof(float), hipMemcpyHostToDevice));
+    A->rho_p = A->rho_alloc + margin;
     std::vector<float>().swap(rp);
     A->support = dalloc<unsigned char>(n);
     HIPCHK(hipMemcpy(A->support, A->h_support.data(), n, hipMemcpyHostToDevice));
@@ -1248,7 +1336,7 @@ void read_counters(prc_projector* A) {
 }
 
 void release(prc_projector* A) {
-    dfree(A->dv), dfree(A->rho_p), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
+    dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total);
@@ -1261,8 +1349,8 @@ void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, cons
                  hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
     ProfScope ps("forward", s);
-    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
-    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    if (wbytes == 2) k_forward<__half, kFwdUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    else k_forward<float, kFwdUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
     HIPCHK(hipGetLastError());
 }
 
@@ -1270,8 +1358,8 @@ void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, cons
                  hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
     ProfScope ps("adjoint", s);
-    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
-    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    if (wbytes == 2) k_adjoint<__half, kAdjUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    else k_adjoint<float, kAdjUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
     HIPCHK(hipGetLastError());
 }
 
