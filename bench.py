@@ -110,7 +110,8 @@ def main():
         st = A.stats
         fp32 = dict(vols_per_s=args.steps / t, ms_per_volume=1e3 * t / args.steps, weight_bytes=st["weight_bytes"],
                     weight_entries=st["weight_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
-                    n_active=st["n_active"])
+                    n_active=st["n_active"], deposits_far=st["deposits"], n_rows=st["n_rows"],
+                    target_entries=st["target_entries"])
         est32 = S.estimate()
         log(f"plain fp32: {fp32['vols_per_s']:.3f} vols/s, weights {st['weight_bytes'] / 2**30:.2f} GiB")
         if not args.no_profile:
@@ -138,6 +139,7 @@ def main():
         st = As[0].stats
         mixed = dict(vols_per_s=args.batch * args.steps / t, ms_per_step=1e3 * t / args.steps,
                      weight_bytes_per_volume=st["weight_bytes"], weight_entries=st["weight_entries"],
+                     target_entries=st["target_entries"],
                      vrs_near=st["vrs_near"], vrs_far=st["vrs_far"])
         est_mixed = B.estimate(0)
         log(f"mixed+batched: {mixed['vols_per_s']:.3f} vols/s")
@@ -167,9 +169,12 @@ def main():
     if k:
         name = max(k, key=lambda kk: k[kk]["total_ms"])
         vols = args.batch if mixed is not None else 1
-        wbytes = (mixed["weight_bytes_per_volume"] if mixed is not None else fp32["weight_bytes"]) * vols
+        st0 = mixed if mixed is not None else fp32
+        b_w = 2 if mixed is not None else 4
+        wbytes = st0["weight_entries"] * b_w * vols     # source-major planes (the adjoint's stream)
+        wbytes_t = st0["target_entries"] * b_w * vols   # target-major planes (the forward's stream)
         n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
-        per_launch = {"forward": wbytes + vols * (4 * n + 4 * n),
+        per_launch = {"forward": wbytes_t + vols * (4 * n + 4 * n),
                       "adjoint": wbytes + vols * (4 * n + 8 * n_act + n)}
         if name in per_launch:
             mean_s = k[name]["mean_ms"] / 1e3

@@ -72,8 +72,9 @@ typedef struct {
     int64_t n_voxels;
     int64_t n_active;        /* voxels with rho >= support_rho */
     int64_t n_rows;          /* X rows holding at least one active voxel */
-    int64_t weight_entries;  /* stored system-matrix entries (row-plane layout, incl. padding) */
-    int64_t weight_bytes;    /* device bytes of the stored weights */
+    int64_t weight_entries;  /* source-major stored entries (the adjoint's planes, incl. padding) */
+    int64_t target_entries;  /* target-major stored entries (the forward's planes, incl. padding) */
+    int64_t weight_bytes;    /* device bytes of both stored copies */
     int64_t vrs_near;        /* voxel-ray steps of the near-block trace (normalisation) */
     int64_t vrs_far;         /* voxel-ray steps of one far-field pass over all sources */
     int64_t deposits;        /* nonzero stored far-field weights */

@@ -47,7 +47,8 @@ class SolveParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("n_voxels", C.c_int64), ("n_active", C.c_int64), ("n_rows", C.c_int64),
-                ("weight_entries", C.c_int64), ("weight_bytes", C.c_int64), ("vrs_near", C.c_int64),
+                ("weight_entries", C.c_int64), ("target_entries", C.c_int64), ("weight_bytes", C.c_int64),
+                ("vrs_near", C.c_int64),
                 ("vrs_far", C.c_int64), ("deposits", C.c_int64), ("norm_seconds", C.c_double),
                 ("cutoff_mm", C.c_float)]
 

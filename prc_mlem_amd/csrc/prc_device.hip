@@ -44,8 +44,6 @@ constexpr int kNearMax = 8;              // a near ray visits at most 7 block vo
 constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
 constexpr int kWave = 64;
 constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
-constexpr int kFwdUnroll = 4;            // forward steps resolved per batch of loads
-constexpr int kAdjUnroll = 8;            // adjoint steps per batch of loads
 constexpr int kFarGroup = 4;             // far-march steps whose density loads are issued together
 
 // ---------------------------------------------------------------------------------------
@@ -162,10 +160,13 @@ struct Vol {            // device descriptor of one volume
     float* W;           // per voxel
     const int* row_xa;  // [ny*nz]
     const int* row_len; // [ny*nz], 0 = no active voxel
-    RowInfo* rinfo;     // [ny*nz][n_rays + 1]
-    void* wbuf;         // weights (float or __half)
+    RowInfo* rinfo;     // [ny*nz][n_rays + 1] source-major layout (planes over source x)
+    RowInfo* tinfo;     // [ny*nz][n_rays + 1] target-major layout (planes over target x)
+    void* wbuf;         // source-major weights (float or __half): the adjoint's stream
+    void* tbuf;         // target-major weights: the forward's stream
     const FarStep* fstep;   // [n_rays][L]
     const int4* pstep;      // [n_rays + 1][LP] (dx, dy, dz) of each step; p = 0 is the near block
+    const int* plin;        // [n_rays + 1][LP] linear voxel offset (dz*ny + dy)*nx + dx of each step
     const int* kfirst;      // [n_rays + 1] first stored step
     const NearStep* nstep;  // [kNearRays][kNearMax]
     const int* nlen;        // [kNearRays]
@@ -299,7 +300,12 @@ struct Walk {
     }
 };
 
-__global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* kfirst, NearStep* nstep, int* nlen) {
+__device__ inline int linear_offset(const Vol& V, int i, int j, int k) {
+    const long long v = ((long long)k * V.ny + j) * V.nx + i;  // steps that are ever used stay inside the grid
+    return (int)max(min(v, (long long)INT_MAX), (long long)INT_MIN);
+}
+
+__global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirst, NearStep* nstep, int* nlen) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     const int nxp = V.nx + 2, nyp = V.ny + 2;
     if (id < V.n_rays) {
@@ -311,6 +317,7 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* kfirst, NearStep
             if (kf < 0 && !w.in_block()) kf = s;
             const int i = w.i, j = w.j, k = w.k;
             pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
+            plin[(size_t)(1 + id) * V.LP + s] = linear_offset(V, i, j, k);
             const float seg = w.step();
             fstep[(size_t)id * V.L + s] = FarStep{seg, (k * nyp + j) * nxp + i};
         }
@@ -333,6 +340,7 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* kfirst, NearStep
     } else if (id < V.n_rays + kNearRays + kNQ) {
         const int q = id - V.n_rays - kNearRays;
         pstep[q] = make_int4(q % 5 - 2, q / 5 % 5 - 2, q / 25 - 2, 0);
+        plin[q] = linear_offset(V, q % 5 - 2, q / 5 % 5 - 2, q / 25 - 2);
         if (q == 0) kfirst[0] = 0;
     }
 }
@@ -444,12 +452,21 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     WT* wp = (WT*)V.wbuf + ri.off + (x0 - xa);
     const float nscale = (float)V.n_rays / (float)kNearRays;
     float wacc = 0.0f;
+    const int R1 = V.n_rays + 1;
+    WT* tb = (WT*)V.tbuf;
 #pragma unroll
     for (int q = 0; q < kNQ; ++q) {
         const WT h = WeightIO<WT>::store_value(act ? bin(q) * nscale : 0.0f);
         const float wq = WeightIO<WT>::to_float(h);
         if (wq > 0.0f) wacc += wq;
         if (valid) gstore(wp + (size_t)q * len + lane, h);
+        // The same weight in the target's planes: bin q of source s is entry q of target s + d_q
+        // (a deposit implies an active target, inside its row's active range).
+        const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
+        if (__any(wq > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
+            const RowInfo ti = gload(V.tinfo + ((size_t)tz * V.ny + ty) * R1);
+            if (wq > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x0 + lane + (q % 5 - 2) - ti.xa), h);
+        }
     }
     if (act) gstore(V.W + loc, wacc);
     if (vrs) {
@@ -518,6 +535,13 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
                     const float wv = WeightIO<WT>::to_float(h);
                     if (wv > 0.0f) wacc += wv, ++deposits;
                     if (valid) gstore(wp + (size_t)su * len + lane, h);
+                    // Target-major copy: step su of this source is step su of target s + D.
+                    if (__any(wv > 0.0f)) {
+                        const int4 d = gload(V.pstep + (size_t)(1 + r) * V.LP + su);
+                        const RowInfo ti = gload(V.tinfo + ((size_t)(z + d.z) * V.ny + (y + d.y)) * R1 + 1 + r);
+                        if (wv > 0.0f)
+                            gstore((WT*)V.tbuf + ti.off + (size_t)(su - kf) * ti.len + (x0 + lane + d.x - ti.xa), h);
+                    }
                 }
             }
             if (!__any(st.alive)) {
@@ -637,6 +661,35 @@ __global__ void k_forward_bounds(Vol V, const int* kglob) {
     V.rinfo[id].kt = kt;
 }
 
+// Target-major layout: row t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
+// source deposits into it (the near block: all 125 bins); planes over the row's active range.
+__global__ void k_target_totals(Vol V, long long* row_tot) {
+    const int rows = V.ny * V.nz, R1 = V.n_rays + 1;
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const int len = gload(V.row_len + row), xa = gload(V.row_xa + row);
+    long long off = 0;
+    for (int p = 0; p < R1; ++p) {
+        const int kt = V.rinfo[(size_t)row * R1 + p].kt;
+        const int cnt = len == 0 ? 0 : (p == 0 ? kNQ : max(0, kt - gload(V.kfirst + p)));
+        RowInfo* e = V.tinfo + (size_t)row * R1 + p;
+        e->cnt = cnt;
+        e->kt = kt;
+        e->off = off;
+        e->xa = xa;
+        e->len = len;
+        off += (long long)cnt * len;
+    }
+    row_tot[row] = off;
+}
+
+__global__ void k_target_offsets(Vol V, const long long* __restrict__ row_base) {
+    const int R1 = V.n_rays + 1;
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long long)V.ny * V.nz * R1) return;
+    V.tinfo[id].off += row_base[id / R1];
+}
+
 // ---------------------------------------------------------------------------------------
 // Projector passes.
 
@@ -646,12 +699,45 @@ __device__ inline float forward_scale(float f, float w) {
     return s > 0.0f ? s : 0.0f;
 }
 
-// out = A g where g already holds f / W (prc.cu:239-254 as a gather). Every voxel of the
-// item is written (zeros where nothing arrives), so out needs no clearing. U steps are
-// resolved on the scalar unit first without branches (source row, plane base, valid lane
-// range), then their 2U vector loads are issued together: the loads depend on geometry only.
-// Lanes outside the valid range load a clamped in-range element and are discarded by a select.
-template <class WT, int U>
+// Eight step offsets of one ray in a single scalar load (the offset table carries 8 entries of
+// slack past its end); entries past `count` are replaced by 0 so every gather stays in bounds.
+__device__ inline void load_offsets8(const int* pl, int j, int count, int (&dl)[8]) {
+    const v8i d = *(const GLOBAL_AS v8i*)(pl + j);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) dl[u] = j + u < count ? d[u] : 0;
+}
+
+// Shared body of both operators: acc += sum over the stored steps of one ray of
+// w[step][lane] * vec[step shift + lane], where `wrow` is the ray's first plane at lane 0,
+// `len` the plane stride, and vec0 + sign * D(step) the uniform gather base.
+template <class WT>
+__device__ inline float ray_planes(const WT* wrow, int len, int count, const int* pl, const float* vec0, int sign,
+                                   int lane, float acc) {
+    for (int j = 0; j < count; j += 8) {
+        int dl[8];
+        load_offsets8(pl, j, count, dl);
+        float w[8], v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const WT* wp = wrow + min(j + u, count - 1) * len;  // uniform plane base
+            const float* vp = vec0 + sign * dl[u];              // uniform gather base
+            w[u] = WeightIO<WT>::load(wp + lane);
+            v[u] = gload(vp + lane);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const bool m = (j + u < count) & (w[u] != 0.0f);
+            acc = m ? fmaf(w[u], v[u], acc) : acc;
+        }
+    }
+    return acc;
+}
+
+// out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
+// streamed; for step (p, k) lane t gathers g at t - D(p,k), a wave-uniform shift. The input
+// vector carries a margin wide enough for every stored shift, so no gather needs a bounds
+// check; zero weights (no source) are skipped by a select. Every voxel of the item is written.
+template <class WT>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ gin,
                                                   float* __restrict__ out) {
@@ -660,73 +746,32 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
     const int4 it = items[item_id];
     if (it.w <= 0) return;
     const Vol& V = vols[it.x];
-    const int nx = V.nx, ny = V.ny, nz = V.nz, R1 = V.n_rays + 1, LP = V.LP;
-    const RowInfo* rinfo = V.rinfo;
-    const int4* pstep = V.pstep;
+    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
+    const RowInfo* tinfo = V.tinfo;
+    const int* plin = V.plin;
     const int* kfirst = V.kfirst;
-    const WT* wb = (const WT*)V.wbuf;
+    const WT* tb = (const WT*)V.tbuf;
     const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    const int row = it.y, x0 = it.z;
     float* o = out + V.vbase + (size_t)row * nx + x0;
-    const RowInfo own = gload(rinfo + (size_t)row * R1);
+    const RowInfo own = gload(tinfo + (size_t)row * R1);
     if (own.len == 0 || x0 + it.w <= own.xa || x0 >= own.xa + own.len) {
         if (lane < it.w) o[lane] = 0.0f;
         return;
     }
-    const float* g = gin + V.vbase;
+    const float* g0 = gin + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
-        const int kf = gload(kfirst + p);
-        const int kend = gload(rinfo + (size_t)row * R1 + p).kt;
-        const int4* ps = pstep + (size_t)p * LP;
-        for (int k = kf; k < kend; k += U) {
-            int4 d[U];
-            int srow[U];
-            bool ok[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) d[u] = gload(ps + min(k + u, kend - 1));
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int ys = y - d[u].y, zs = z - d[u].z;
-                ok[u] = (k + u < kend) & ((unsigned)ys < (unsigned)ny) & ((unsigned)zs < (unsigned)nz);
-                srow[u] = ok[u] ? zs * ny + ys : row;
-            }
-            RowInfo ri[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) ri[u] = gload(rinfo + (size_t)srow[u] * R1 + p);
-            const WT* wptr[U];
-            const float* gptr[U];
-            int lo[U], hi[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int xs0 = x0 - d[u].x;  // source x of lane 0
-                const bool live = ok[u] & (k + u - kf < ri[u].cnt) & (ri[u].len > 0);
-                lo[u] = live ? ri[u].xa - xs0 : 0;
-                hi[u] = live ? lo[u] + ri[u].len : 0;
-                wptr[u] = live ? wb + ri[u].off + (size_t)(k + u - kf) * ri[u].len + (xs0 - ri[u].xa) : wb;
-                gptr[u] = live ? g + (size_t)srow[u] * nx + xs0 : g;
-            }
-            float w[U], gv[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int lc = max(lo[u], min(lane, hi[u] - 1));  // in range; lane 0 of wb / g when empty
-                w[u] = WeightIO<WT>::load(wptr[u] + lc);
-                gv[u] = gload(gptr[u] + lc);
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool m = (lane >= lo[u]) & (lane < hi[u]) & (w[u] != 0.0f);
-                acc = m ? fmaf(w[u], gv[u], acc) : acc;
-            }
-        }
+        const RowInfo ti = gload(tinfo + (size_t)row * R1 + p);
+        const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
+        acc = ray_planes(tb + ti.off + (x0 - ti.xa), ti.len, ti.cnt, pl, g0, -1, lane, acc);
     }
     if (lane < it.w) o[lane] = acc;
 }
 
-// out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the planes.
-// The gather of in[] is bounded by the target's position (inside the grid), not by the weight,
-// so U weight loads and U gathers issue together; zero weights are skipped by a select.
-template <class WT, int U>
+// out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
+// planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
+template <class WT>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ in,
                                                   float* __restrict__ out) {
@@ -735,50 +780,22 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     const int4 it = items[item_id];
     if (it.w <= 0) return;
     const Vol& V = vols[it.x];
-    const int nx = V.nx, ny = V.ny, nz = V.nz, R1 = V.n_rays + 1, LP = V.LP;
+    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
     const RowInfo* rinfo = V.rinfo;
-    const int4* pstep = V.pstep;
+    const int* plin = V.plin;
     const int* kfirst = V.kfirst;
     const WT* wb = (const WT*)V.wbuf;
     const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    const int row = it.y, x0 = it.z;
     const bool valid = lane < it.w;
-    const int lw = min(lane, it.w - 1);  // clamped lane for the weight stream
-    const float* src = in + V.vbase;
+    const float* s0 = in + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
         const RowInfo ri = gload(rinfo + (size_t)row * R1 + p);
-        const int kf = gload(kfirst + p);
-        const WT* wp = wb + ri.off + (x0 - ri.xa) + lw;
-        const int4* ps = pstep + (size_t)p * LP + kf;
-        for (int j = 0; j < ri.cnt; j += U) {
-            int4 d[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) d[u] = gload(ps + min(j + u, ri.cnt - 1));
-            const float* sp[U];
-            int lo[U], hi[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int ty = y + d[u].y, tz = z + d[u].z;
-                const bool ok = (j + u < ri.cnt) & ((unsigned)ty < (unsigned)ny) & ((unsigned)tz < (unsigned)nz);
-                lo[u] = ok ? -(x0 + d[u].x) : 0;  // lanes whose target x is inside the grid
-                hi[u] = ok ? lo[u] + nx : 0;
-                sp[u] = ok ? src + ((size_t)tz * ny + ty) * nx + (x0 + d[u].x) : src;
-            }
-            float w[U], v[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                w[u] = WeightIO<WT>::load(wp + (size_t)min(j + u, ri.cnt - 1) * ri.len);
-                v[u] = gload(sp[u] + max(lo[u], min(lane, hi[u] - 1)));
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool m = (j + u < ri.cnt) & (lane >= lo[u]) & (lane < hi[u]) & (w[u] != 0.0f);
-                acc = m ? fmaf(w[u], v[u], acc) : acc;
-            }
-        }
+        const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
+        acc = ray_planes(wb + ri.off + (x0 - ri.xa), ri.len, ri.cnt, pl, s0, 1, lane, acc);
     }
-    const size_t loc = (size_t)row * V.nx + x0 + lw;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     if (valid && gload(V.support + loc)) {
         const float wv = gload(V.W + loc);
         out[V.vbase + loc] = wv > 0.0f ? acc / wv : 0.0f;
@@ -1084,10 +1101,14 @@ struct prc_projector {
     int* row_xa = nullptr;
     int* row_len = nullptr;
     RowInfo* rinfo = nullptr;
+    RowInfo* tinfo = nullptr;
     void* wbuf = nullptr;
-    long long wcap = 0;  // weight buffer capacity (entries)
+    void* tbuf = nullptr;
+    long long wcap = 0, tcap = 0;  // weight buffer capacities (entries)
+    long long margin = -1;         // elements of padding on each side of gathered vectors
     FarStep* fstep = nullptr;
     int4* pstep = nullptr;
+    int* plin = nullptr;
     int* kfirst = nullptr;
     NearStep* nstep = nullptr;
     int* nlen = nullptr;
@@ -1101,7 +1122,8 @@ struct prc_projector {
     std::vector<int4> h_items_act, h_items_all;
     std::vector<unsigned char> h_support;
     unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
-    float* scratch_in = nullptr;
+    float* scratch_in = nullptr;   // margin-padded input of the device-pointer forward
+    float* scratch_adj = nullptr;  // margin-padded input of the device-pointer adjoint
     prc_stats stats{};
     size_t n = 0;
     int wbytes = 4;
@@ -1113,6 +1135,7 @@ struct prc_solver {
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
     float* base[5] = {};  // f, prev, y, hprev, est as allocated (the loop rotates them)
+    float* raw[7] = {};   // allocations: every vector has the projector's gather margin on both sides
     Scal* sc = nullptr;
     Scal sc0{};
     double* partial = nullptr;
@@ -1247,6 +1270,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LP = std::max(L, kNQ);
     A->fstep = dalloc<FarStep>((size_t)R * L);
     A->pstep = dalloc<int4>((size_t)R1 * LP);
+    A->plin = dalloc<int>((size_t)R1 * LP + 8);  // 8 entries of slack for 8-wide scalar loads
+    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + 8) * sizeof(int)));
     A->kfirst = dalloc<int>(R1);
     A->kglob = dalloc<int>(R1);
     A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
@@ -1262,27 +1287,40 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     V.exact = (g.flags & PRC_FLAG_EXACT_RANGE) != 0;
     V.L = L, V.LP = LP, V.vbase = 0;
     V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
-    V.fstep = A->fstep, V.pstep = A->pstep, V.kfirst = A->kfirst, V.nstep = A->nstep, V.nlen = A->nlen;
+    V.fstep = A->fstep, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
+    V.nlen = A->nlen;
 
-    k_rays<<<blocks_for(R + kNearRays + kNQ, 128), 128>>>(V, A->fstep, A->pstep, A->kfirst, A->nstep, A->nlen);
+    k_rays<<<blocks_for(R + kNearRays + kNQ, 128), 128>>>(V, A->fstep, A->pstep, A->plin, A->kfirst, A->nstep,
+                                                          A->nlen);
     HIPCHK(hipGetLastError());
 
     build_items(A);
     V.row_xa = A->row_xa, V.row_len = A->row_len;
     const size_t rows = (size_t)g.ny * g.nz;
     A->rinfo = dalloc<RowInfo>(rows * R1);
+    A->tinfo = dalloc<RowInfo>(rows * R1);
     A->row_tot = dalloc<long long>(rows);
     A->row_base = dalloc<long long>(rows);
-    A->total = dalloc<long long>(1);
-    V.rinfo = A->rinfo;
-    V.wbuf = nullptr;
+    A->total = dalloc<long long>(2);
+    V.rinfo = A->rinfo, V.tinfo = A->tinfo;
+    V.wbuf = nullptr, V.tbuf = nullptr;
     A->dv = dalloc<Vol>(1);
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
 }
 
-// The normalisation pass (Projector::Projector, prc.cu:405-422): count, lay out, then march
-// again writing every weight and W. One host synchronisation (the buffer size); the weight
-// buffer is reused when it is large enough.
+// Grow a weight buffer to hold `entries` (+ one wave of slack for clamped-free tail reads).
+void ensure_weights(void*& buf, long long& cap, long long entries, int wbytes) {
+    const long long need = entries + kWave;
+    if (buf && need <= cap) return;
+    if (buf) (void)hipFree(buf);
+    buf = nullptr, cap = 0;
+    buf = (void*)dalloc<unsigned char>((size_t)need * wbytes);
+    cap = need;
+}
+
+// The normalisation pass (Projector::Projector, prc.cu:405-422): count, lay out both copies of
+// the matrix, then march again writing every weight (source-major for the adjoint, target-major
+// for the forward) and W. One host synchronisation (the buffer sizes); buffers are reused.
 void normalise_projector(prc_projector* A) {
     Vol& V = A->hv;
     const size_t rows = (size_t)V.ny * V.nz;
@@ -1298,26 +1336,29 @@ void normalise_projector(prc_projector* A) {
         k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)rows, A->total);
         k_row_offsets<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->row_base);
         k_kglob<<<R1, 256>>>(V, A->kglob);
-        HIPCHK(hipGetLastError());
-    }
-    long long total = 0;
-    HIPCHK(hipMemcpy(&total, A->total, sizeof(total), hipMemcpyDeviceToHost));
-    if (total > A->wcap || !A->wbuf) {
-        if (A->wbuf) (void)hipFree(A->wbuf);
-        A->wbuf = nullptr;
-        A->wcap = 0;
-        A->wbuf = (void*)dalloc<unsigned char>((size_t)std::max<long long>(total, 1) * A->wbytes);
-        A->wcap = std::max<long long>(total, 1);
-    }
-    A->stats.weight_entries = total;
-    A->stats.weight_bytes = total * A->wbytes;
-    V.wbuf = A->wbuf;
-    HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
-    {
-        ProfScope ps("forward_bounds", 0);
         k_forward_bounds<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->kglob);
+        k_target_totals<<<blocks_for((long long)rows, 256), 256>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)rows, A->total + 1);
+        k_target_offsets<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->row_base);
         HIPCHK(hipGetLastError());
     }
+    long long total[2] = {0, 0};
+    HIPCHK(hipMemcpy(total, A->total, sizeof(total), hipMemcpyDeviceToHost));
+    std::vector<int> kglob(R1);
+    HIPCHK(hipMemcpy(kglob.data(), A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost));
+    // Every stored step moves at most maxk voxels along each axis from its source.
+    const int maxk = *std::max_element(kglob.begin(), kglob.end());
+    const long long margin = (long long)(maxk + 1) * ((long long)V.nx * V.ny + V.nx + 1) + kWave;
+    if (A->margin < 0) A->margin = margin;
+    else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
+    ensure_weights(A->wbuf, A->wcap, total[0], A->wbytes);
+    ensure_weights(A->tbuf, A->tcap, total[1], A->wbytes);
+    A->stats.weight_entries = total[0];
+    A->stats.target_entries = total[1];
+    A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
+    V.wbuf = A->wbuf, V.tbuf = A->tbuf;
+    HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(A->tbuf, 0, (size_t)A->tcap * A->wbytes, 0));
     if (A->wbytes == 2) {
         launch_near<__half>(A);
         launch_far<__half>(A, 2);
@@ -1339,9 +1380,11 @@ void release(prc_projector* A) {
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
-    dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total);
+    dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
+    dfree(A->scratch_adj);
     if (A->wbuf) (void)hipFree(A->wbuf);
-    A->wbuf = nullptr;
+    if (A->tbuf) (void)hipFree(A->tbuf);
+    A->wbuf = A->tbuf = nullptr;
 }
 
 // Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
@@ -1349,8 +1392,8 @@ void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, cons
                  hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
     ProfScope ps("forward", s);
-    if (wbytes == 2) k_forward<__half, kFwdUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
-    else k_forward<float, kFwdUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
     HIPCHK(hipGetLastError());
 }
 
@@ -1358,8 +1401,8 @@ void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, cons
                  hipStream_t s) {
     const unsigned nb = blocks_for(n_items, kItemsPerBlock);
     ProfScope ps("adjoint", s);
-    if (wbytes == 2) k_adjoint<__half, kAdjUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
-    else k_adjoint<float, kAdjUnroll><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
     HIPCHK(hipGetLastError());
 }
 
@@ -1624,21 +1667,30 @@ void prc_projector_destroy(prc_projector* A) {
 int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out) {
     return guarded([&] {
         if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
-        if (!A->scratch_in) A->scratch_in = dalloc<float>(A->n);
+        if (!A->scratch_in) {
+            A->scratch_in = dalloc<float>(A->n + 2 * A->margin);
+            HIPCHK(hipMemset(A->scratch_in, 0, (A->n + 2 * A->margin) * sizeof(float)));
+        }
+        float* g = A->scratch_in + A->margin;  // f / W with the forward's gather margin
         {
             ProfScope ps("extrapolate", 0);
-            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in,
-                                                                    A->scratch_in);
+            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, A->scratch_in, dev_out, 0);
+        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, g, dev_out, 0);
     });
 }
 
 int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out) {
     return guarded([&] {
         if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, dev_in, dev_out, 0);
+        if (!A->scratch_adj) {
+            A->scratch_adj = dalloc<float>(A->n + 2 * A->margin);
+            HIPCHK(hipMemset(A->scratch_adj, 0, (A->n + 2 * A->margin) * sizeof(float)));
+        }
+        float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
+        HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, 0);
     });
 }
 
@@ -1709,8 +1761,13 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
         S->support_sum = support_sum;
         S->sc0 = initial_scalars(A, support_sum, *params);
         try {
-            S->g = dalloc<float>(n), S->gf = dalloc<float>(n);
-            for (int i = 0; i < 5; ++i) S->base[i] = dalloc<float>(n);
+            const size_t padded = n + 2 * (size_t)A->margin;
+            for (int i = 0; i < 7; ++i) {
+                S->raw[i] = dalloc<float>(padded);
+                HIPCHK(hipMemset(S->raw[i], 0, padded * sizeof(float)));
+            }
+            S->g = S->raw[5] + A->margin, S->gf = S->raw[6] + A->margin;
+            for (int i = 0; i < 5; ++i) S->base[i] = S->raw[i] + A->margin;
             HIPCHK(hipMemcpy(S->g, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
             S->sc = dalloc<Scal>(1);
             S->partial = dalloc<double>(2 * (size_t)A->n_all);
@@ -1741,8 +1798,7 @@ int prc_solver_reset(prc_solver* S) {
 
 void prc_solver_destroy(prc_solver* S) {
     if (!S) return;
-    dfree(S->g), dfree(S->gf);
-    for (float*& b : S->base) dfree(b);
+    for (float*& b : S->raw) dfree(b);
     dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
     delete S;
 }
@@ -1838,14 +1894,14 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
                 check_params(&params[v]);
                 prior = prior || params[v].beta > 0.0f;
                 b->vols[v] = A->hv;
-                b->vols[v].vbase = b->total;
+                b->vols[v].vbase = b->total + A->margin;  // each volume keeps its gather margins
                 begin[v] = (int)all.size();
                 for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
                 for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
-                b->total += A->n;
+                b->total += A->n + 2 * (size_t)A->margin;
             }
             begin[n] = (int)all.size();
-            std::vector<float> g(b->total);
+            std::vector<float> g(b->total, 0.0f);
             for (int v = 0; v < n; ++v) {
                 const double ss = preprocess_data(projectors[v], host_g[v], g.data() + b->vols[v].vbase);
                 b->sc0[v] = initial_scalars(projectors[v], ss, params[v]);
@@ -1866,6 +1922,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
             B.g = (float*)own(dalloc<float>(b->total));
             B.gf = (float*)own(dalloc<float>(b->total));
+            HIPCHK(hipMemset(B.gf, 0, b->total * sizeof(float)));
             for (int i = 0; i < 5; ++i) b->base[i] = (float*)own(dalloc<float>(b->total));
             HIPCHK(hipMemcpy(B.g, g.data(), b->total * sizeof(float), hipMemcpyHostToDevice));
             B.sc = (Scal*)own(dalloc<Scal>(n));
