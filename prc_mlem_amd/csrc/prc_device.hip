@@ -27,6 +27,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -1102,8 +1103,10 @@ struct prc_projector {
     int* row_len = nullptr;
     RowInfo* rinfo = nullptr;
     RowInfo* tinfo = nullptr;
-    void* wbuf = nullptr;
-    void* tbuf = nullptr;
+    void* wraw = nullptr;  // allocations of the two weight copies (with slack)
+    void* traw = nullptr;
+    void* wbuf = nullptr;  // source-major planes = wraw + one wave
+    void* tbuf = nullptr;  // target-major planes = traw + one wave
     long long wcap = 0, tcap = 0;  // weight buffer capacities (entries)
     long long margin = -1;         // elements of padding on each side of gathered vectors
     FarStep* fstep = nullptr;
@@ -1308,13 +1311,17 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
 }
 
-// Grow a weight buffer to hold `entries` (+ one wave of slack for clamped-free tail reads).
-void ensure_weights(void*& buf, long long& cap, long long entries, int wbytes) {
-    const long long need = entries + kWave;
-    if (buf && need <= cap) return;
-    if (buf) (void)hipFree(buf);
-    buf = nullptr, cap = 0;
-    buf = (void*)dalloc<unsigned char>((size_t)need * wbytes);
+// Grow a weight buffer to hold `entries`, with one wave of slack on both sides: the operators
+// read whole 64-lane segments, and a forward segment may start up to 63 voxels before its row's
+// active range (x0 - xa >= -63) or end up to 63 past the last plane. `raw` is the allocation,
+// `buf` = raw + kWave elements.
+void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, int wbytes) {
+    const long long need = entries;
+    if (raw && need <= cap) return;
+    if (raw) (void)hipFree(raw);
+    raw = buf = nullptr, cap = 0;
+    raw = (void*)dalloc<unsigned char>((size_t)(need + 2 * kWave) * wbytes);
+    buf = (unsigned char*)raw + (size_t)kWave * wbytes;
     cap = need;
 }
 
@@ -1351,14 +1358,14 @@ void normalise_projector(prc_projector* A) {
     const long long margin = (long long)(maxk + 1) * ((long long)V.nx * V.ny + V.nx + 1) + kWave;
     if (A->margin < 0) A->margin = margin;
     else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
-    ensure_weights(A->wbuf, A->wcap, total[0], A->wbytes);
-    ensure_weights(A->tbuf, A->tcap, total[1], A->wbytes);
+    ensure_weights(A->wraw, A->wbuf, A->wcap, total[0], A->wbytes);
+    ensure_weights(A->traw, A->tbuf, A->tcap, total[1], A->wbytes);
     A->stats.weight_entries = total[0];
     A->stats.target_entries = total[1];
     A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
     V.wbuf = A->wbuf, V.tbuf = A->tbuf;
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(A->tbuf, 0, (size_t)A->tcap * A->wbytes, 0));
+    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, 0));
     if (A->wbytes == 2) {
         launch_near<__half>(A);
         launch_far<__half>(A, 2);
@@ -1366,6 +1373,51 @@ void normalise_projector(prc_projector* A) {
         launch_near<float>(A);
         launch_far<float>(A, 2);
     }
+}
+
+// Host replay of every index range the forward and adjoint read (PRC_AUDIT=1): plane reads
+// must stay inside the buffer's one-wave slack, offset-table reads inside its 8 spare entries,
+// and every gather shift inside the vectors' margin. Used by the tests before any operator
+// launch; fails instead of launching a kernel that could read out of bounds.
+void audit_layout(prc_projector* A) {
+    const Vol& V = A->hv;
+    const size_t rows = (size_t)V.ny * V.nz;
+    const int R1 = V.n_rays + 1, LP = V.LP;
+    std::vector<RowInfo> ri(rows * R1), ti(rows * R1);
+    std::vector<int> kf(R1), pl((size_t)R1 * LP + 8);
+    HIPCHK(hipMemcpy(ri.data(), A->rinfo, ri.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ti.data(), A->tinfo, ti.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(kf.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pl.data(), A->plin, pl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    const long long shift_max = A->margin - kWave;
+    auto check_item = [&](const std::vector<RowInfo>& info, long long total, int row, int x0, const char* what) {
+        for (int p = 0; p < R1; ++p) {
+            const RowInfo& e = info[(size_t)row * R1 + p];
+            if (e.cnt <= 0) continue;
+            const long long lo = e.off + (x0 - e.xa);
+            const long long hi = e.off + (long long)(e.cnt - 1) * e.len + (x0 - e.xa) + kWave - 1;
+            if (lo < -kWave || hi >= total + kWave)
+                fail(PRC_ERR_INVALID, "audit: %s planes of row %d ray %d read [%lld, %lld] outside [%d, %lld)", what,
+                     row, p, lo, hi, -kWave, total + kWave);
+            const long long last = (long long)p * LP + kf[p] + ((e.cnt - 1) / 8) * 8 + 7;
+            if (kf[p] + e.cnt > LP || last >= (long long)pl.size())
+                fail(PRC_ERR_INVALID, "audit: %s offsets of ray %d read past the table", what, p);
+            for (int j = 0; j < e.cnt; ++j) {
+                const long long d = pl[(size_t)p * LP + kf[p] + j];
+                if (d > shift_max || d < -shift_max)
+                    fail(PRC_ERR_INVALID, "audit: %s shift %lld of ray %d exceeds the margin", what, d, p);
+            }
+        }
+    };
+    for (const int4& it : A->h_items_all) {
+        if (it.w <= 0) continue;
+        const RowInfo& own = ti[(size_t)it.y * R1];
+        if (own.len == 0 || it.z + it.w <= own.xa || it.z >= own.xa + own.len) continue;
+        if (it.z - own.xa < -(kWave - 1)) fail(PRC_ERR_INVALID, "audit: forward segment starts too far left");
+        check_item(ti, A->stats.target_entries, it.y, it.z, "forward");
+    }
+    for (const int4& it : A->h_items_act)
+        if (it.w > 0) check_item(ri, A->stats.weight_entries, it.y, it.z, "adjoint");
 }
 
 void read_counters(prc_projector* A) {
@@ -1382,9 +1434,9 @@ void release(prc_projector* A) {
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
     dfree(A->scratch_adj);
-    if (A->wbuf) (void)hipFree(A->wbuf);
-    if (A->tbuf) (void)hipFree(A->tbuf);
-    A->wbuf = A->tbuf = nullptr;
+    if (A->wraw) (void)hipFree(A->wraw);
+    if (A->traw) (void)hipFree(A->traw);
+    A->wraw = A->traw = A->wbuf = A->tbuf = nullptr;
 }
 
 // Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
@@ -1642,6 +1694,8 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
             normalise_projector(A);
             HIPCHK(hipDeviceSynchronize());
             A->stats.norm_seconds = now_s() - t0;
+            const char* audit = getenv("PRC_AUDIT");
+            if (audit && *audit == '1') audit_layout(A);
         } catch (...) {
             release(A);
             delete A;

@@ -7,6 +7,8 @@ import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 sys.path.insert(0, str(ROOT))
+# Every projector built by the tests replays the operators' index ranges on the host first.
+os.environ.setdefault("PRC_AUDIT", "1")
 
 
 def pytest_configure(config):
