@@ -767,7 +767,10 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
         const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
         acc = ray_planes(tb + ti.off + (x0 - ti.xa), ti.len, ti.cnt, pl, g0, -1, lane, acc);
     }
-    if (lane < it.w) o[lane] = acc;
+    // Lanes outside the row's active range read neighbouring planes: they are off support and
+    // receive nothing (prc.cu:432 clears out; deposits land on support voxels only).
+    const int x = x0 + lane;
+    if (lane < it.w) o[lane] = (x >= own.xa && x < own.xa + own.len) ? acc : 0.0f;
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
