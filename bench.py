@@ -111,7 +111,7 @@ def main():
         fp32 = dict(vols_per_s=args.steps / t, ms_per_volume=1e3 * t / args.steps, weight_bytes=st["weight_bytes"],
                     weight_entries=st["weight_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
                     n_active=st["n_active"], deposits_far=st["deposits"], n_rows=st["n_rows"],
-                    target_entries=st["target_entries"])
+                    target_entries=st["target_entries"], near_uniform=st["near_uniform"])
         est32 = S.estimate()
         log(f"plain fp32: {fp32['vols_per_s']:.3f} vols/s, weights {st['weight_bytes'] / 2**30:.2f} GiB")
         if not args.no_profile:
@@ -191,7 +191,11 @@ def main():
         log(f"cpu baseline: oracle on a {args.cpu_slices}-slice slab")
         t_slab, vrs_slab, cores = cpu_baseline(rho, datas[0], args.cpu_slices, 30)
         rate = vrs_slab / t_slab
-        vrs_full = fp32["vrs_near"] + (1 + 2 * ITERS) * fp32["vrs_far"]
+        # The oracle marches every near block: extrapolate to the full march's step count.
+        full = prc.Projector(rho, SPACING, ISOTOPE, RAYS, near_memo=False)
+        vrs_near_all = full.stats["vrs_near"]
+        full.close()
+        vrs_full = vrs_near_all + (1 + 2 * ITERS) * fp32["vrs_far"]
         cpu = {"value": round(rate / vrs_full, 6), "unit": "corrected vols/s", "cores": cores, "kind": "port",
                "sample": f"FP32 oracle: normalisation + {ITERS} iterations on a {args.cpu_slices}-slice Z slab "
                          f"({t_slab:.1f} s, {vrs_slab:.3g} voxel-ray steps); vols/s extrapolated from its "

@@ -40,7 +40,9 @@ enum {
 /* Projector flags. */
 enum {
     PRC_FLAG_EXACT_RANGE = 1,  /* --exact-range: analytic survival per step (prc.cu:75, 625) */
-    PRC_FLAG_NO_CACHE_NEAR = 2 /* --no-cache-near: accepted; weights are always stored once */
+    PRC_FLAG_NO_CACHE_NEAR = 2, /* --no-cache-near: accepted; weights are always stored once */
+    PRC_FLAG_NO_NEAR_MEMO = 4   /* march every source's near block, even where 5x5x5 uniform
+                                   neighbourhoods would share one (same result, for testing) */
 };
 
 typedef struct prc_projector prc_projector;
@@ -78,6 +80,7 @@ typedef struct {
     int64_t vrs_near;        /* voxel-ray steps of the near-block trace (normalisation) */
     int64_t vrs_far;         /* voxel-ray steps of one far-field pass over all sources */
     int64_t deposits;        /* nonzero stored far-field weights */
+    int64_t near_uniform;    /* sources whose near block was copied from a uniform-density template */
     double norm_seconds;     /* wall time of the normalisation build */
     float cutoff_mm;         /* water-equivalent range cutoff (prc.cu:83-91, 587) */
 } prc_stats;

@@ -23,6 +23,7 @@ PRC_EXE = LIB_DIR / "prc"
 PRECISION = {"fp32": 0, "mixed": 1}
 FLAG_EXACT_RANGE = 1
 FLAG_NO_CACHE_NEAR = 2
+FLAG_NO_NEAR_MEMO = 4
 RANGE_BINS = 4096
 NEAR_RAYS = 2048
 
@@ -49,7 +50,8 @@ class Stats(C.Structure):
     _fields_ = [("n_voxels", C.c_int64), ("n_active", C.c_int64), ("n_rows", C.c_int64),
                 ("weight_entries", C.c_int64), ("target_entries", C.c_int64), ("weight_bytes", C.c_int64),
                 ("vrs_near", C.c_int64),
-                ("vrs_far", C.c_int64), ("deposits", C.c_int64), ("norm_seconds", C.c_double),
+                ("vrs_far", C.c_int64), ("deposits", C.c_int64), ("near_uniform", C.c_int64),
+                ("norm_seconds", C.c_double),
                 ("cutoff_mm", C.c_float)]
 
 
@@ -154,7 +156,7 @@ class Projector:
     """
 
     def __init__(self, rho, spacing, isotope="68Ga", n_rays=128, support_rho=0.05, air_cap_mm=50.0,
-                 precision="fp32", exact_range=False, table=None):
+                 precision="fp32", exact_range=False, table=None, near_memo=True):
         rho = _volume(rho)
         if rho.ndim != 3:
             raise ValueError("rho must be a 3D (NZ, NY, NX) volume")
@@ -162,7 +164,7 @@ class Projector:
         nz, ny, nx = rho.shape
         self.geometry = Geometry(nx, ny, nz, *map(float, spacing), isotope_index(isotope), int(n_rays),
                                  float(support_rho), float(air_cap_mm), PRECISION[precision],
-                                 FLAG_EXACT_RANGE if exact_range else 0)
+                                 (FLAG_EXACT_RANGE if exact_range else 0) | (0 if near_memo else FLAG_NO_NEAR_MEMO))
         self.precision = precision
         h = C.c_void_p()
         tab = None if table is None else np.ascontiguousarray(table, np.float32)

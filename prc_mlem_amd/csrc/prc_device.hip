@@ -384,26 +384,17 @@ __device__ inline void load_table(float* tab, const Vol& V) {
     __syncthreads();
 }
 
-// Near block: 2048 rays per source, 125 bins accumulated in registers (bin index is uniform).
-template <bool EXACT, class WT>
-__global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                               int n_items, unsigned long long* vrs) {
-    __shared__ float tab[kBins + 1];
-    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
-    const int4 it = items[min(item_id, n_items - 1)];
-    const Vol& V = vols[it.x];
-    if (!EXACT) load_table(tab, V);
-    if (item_id >= n_items || it.w <= 0) return;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
-    const bool valid = lane < it.w;
-    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
-    const bool act = valid && gload(V.support + loc);
-    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
-    unsigned long long visits = 0;
-
-    // Shell accumulators (shell 4 split in two halves of 18 to stay within register indexing).
-    float a0 = 0.0f, a1[6], a2[18], a3[32], a4l[18], a4h[18], a5[24], a6[8];
+// The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into the shell
+// registers (shell 4 split in two halves of 18 to stay within register indexing). rho_at(dp)
+// returns the density at padded offset dp from the lane's source.
+#define NEAR_ACCUMULATORS float a0 = 0.0f, a1[6], a2[18], a3[32], a4l[18], a4h[18], a5[24], a6[8]
+#define NEAR_ARGS a0, a1, a2, a3, a4l, a4h, a5, a6
+template <bool EXACT, class RhoAt>
+__device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, const float* tab, bool act, RhoAt&& rho_at,
+                                                              float& a0, float (&a1)[6], float (&a2)[18],
+                                                              float (&a3)[32], float (&a4l)[18], float (&a4h)[18],
+                                                              float (&a5)[24], float (&a6)[8],
+                                                              unsigned long long& visits) {
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
         if (i < 6) a1[i] = 0.0f;
@@ -422,7 +413,7 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
 #pragma unroll
         for (int s = 0; s < 7; ++s) ns[s] = s < len ? gload(nsr + s) : NearStep{0.0f, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 7; ++s) rv[s] = (s < len && act) ? gload(rho0 + ns[s].dp + lane) : INFINITY;
+        for (int s = 0; s < 7; ++s) rv[s] = (s < len && act) ? rho_at(ns[s].dp) : INFINITY;
         Lane st;
         st.alive = act;
         auto step = [&](int s) {
@@ -442,27 +433,52 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
         if (len > 5) { const float d = step(5); a5[ns[5].loc] += d; }
         if (len > 6) { const float d = step(6); a6[ns[6].loc] += d; }
     }
-    auto bin = [&](int q) {
-        const int sh = kShellMap.shell[q], lc = kShellMap.local[q];
-        return sh == 0 ? a0 : sh == 1 ? a1[lc] : sh == 2 ? a2[lc] : sh == 3 ? a3[lc]
-             : sh == 4 ? (lc < 18 ? a4l[lc] : a4h[lc - 18]) : sh == 5 ? a5[lc] : a6[lc];
-    };
+}
+
+#define NEAR_BIN(q)                                                                                      \
+    (kShellMap.shell[q] == 0   ? a0                                                                     \
+     : kShellMap.shell[q] == 1 ? a1[kShellMap.local[q]]                                                 \
+     : kShellMap.shell[q] == 2 ? a2[kShellMap.local[q]]                                                 \
+     : kShellMap.shell[q] == 3 ? a3[kShellMap.local[q]]                                                 \
+     : kShellMap.shell[q] == 4 ? (kShellMap.local[q] < 18 ? a4l[kShellMap.local[q]] : a4h[kShellMap.local[q] - 18]) \
+     : kShellMap.shell[q] == 5 ? a5[kShellMap.local[q]]                                                 \
+                               : a6[kShellMap.local[q]])
+
+// Near block over row segments: one wave = 64 consecutive sources of a row (density reads are
+// uniform base + lane). Writes the 125 weights of every source to its own planes and to its
+// targets' planes (bin q of source s is entry q of target s + d_q), and starts W with their sum.
+template <bool EXACT, class WT>
+__global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                               int n_items, unsigned long long* vrs) {
+    __shared__ float tab[kBins + 1];
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    const int4 it = items[min(item_id, n_items - 1)];
+    const Vol& V = vols[it.x];
+    if (!EXACT) load_table(tab, V);
+    if (item_id >= n_items || it.w <= 0) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const bool act = valid && gload(V.support + loc);
+    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
+    unsigned long long visits = 0;
+    NEAR_ACCUMULATORS;
+    near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rho0 + dp + lane); }, NEAR_ARGS, visits);
     // Near weights on the scale of one far ray (prc.cu:205-218); W starts with their sum.
-    const RowInfo ri = gload(V.rinfo + (size_t)row * (V.n_rays + 1));
-    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
-    WT* wp = (WT*)V.wbuf + ri.off + (x0 - xa);
+    const int R1 = V.n_rays + 1;
+    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
+    WT* wp = (WT*)V.wbuf + ri.off + (x0 - ri.xa);
+    WT* tb = (WT*)V.tbuf;
     const float nscale = (float)V.n_rays / (float)kNearRays;
     float wacc = 0.0f;
-    const int R1 = V.n_rays + 1;
-    WT* tb = (WT*)V.tbuf;
 #pragma unroll
     for (int q = 0; q < kNQ; ++q) {
-        const WT h = WeightIO<WT>::store_value(act ? bin(q) * nscale : 0.0f);
+        const WT h = WeightIO<WT>::store_value(act ? NEAR_BIN(q) * nscale : 0.0f);
         const float wq = WeightIO<WT>::to_float(h);
         if (wq > 0.0f) wacc += wq;
-        if (valid) gstore(wp + (size_t)q * len + lane, h);
-        // The same weight in the target's planes: bin q of source s is entry q of target s + d_q
-        // (a deposit implies an active target, inside its row's active range).
+        if (valid) gstore(wp + (size_t)q * ri.len + lane, h);
+        // A deposit implies an active target, inside its row's active range.
         const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
         if (__any(wq > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
             const RowInfo ti = gload(V.tinfo + ((size_t)tz * V.ny + ty) * R1);
@@ -474,6 +490,179 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
         const long long tot = wave_sum_ll((long long)visits);
         if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
     }
+}
+
+// Near block over a list of sources (voxel index, template id or -1), 64 arbitrary sources per
+// wave with per-lane density addressing: the sources whose 5x5x5 neighbourhood is not one
+// uniform density, plus one representative per uniform density, whose 125 weights and sum
+// become that density's template (every source with the same uniform neighbourhood computes the
+// same weights bit for bit, so the others copy them in k_near_fill).
+template <bool EXACT, class WT>
+__global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols, const int2* __restrict__ list,
+                                                    int n_list, float* __restrict__ tmpl,
+                                                    float* __restrict__ tmpl_sum, unsigned long long* vrs) {
+    __shared__ float tab[kBins + 1];
+    const Vol& V = vols[0];
+    if (!EXACT) load_table(tab, V);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n_list;
+    const int2 e = list[act ? i : n_list - 1];
+    const int nx = V.nx, ny = V.ny;
+    const int v = e.x, x = v % nx, row = v / nx, y = row % ny, z = row / ny;
+    const float* rb = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + x + 1;
+    unsigned long long visits = 0;
+    NEAR_ACCUMULATORS;
+    near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rb + dp); }, NEAR_ARGS, visits);
+    if (vrs) {
+        const long long tot = wave_sum_ll((long long)visits);
+        if ((threadIdx.x & (kWave - 1)) == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
+    }
+    if (!act) return;
+    const int R1 = V.n_rays + 1;
+    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
+    WT* wp = (WT*)V.wbuf + ri.off + (x - ri.xa);
+    WT* tb = (WT*)V.tbuf;
+    const float nscale = (float)V.n_rays / (float)kNearRays;
+    float wacc = 0.0f;
+#pragma unroll
+    for (int dz = -2; dz <= 2; ++dz)
+#pragma unroll
+        for (int dy = -2; dy <= 2; ++dy) {
+            const int ty = y + dy, tz = z + dz;
+            RowInfo ti{};
+            if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)V.nz) ti = gload(V.tinfo + ((size_t)tz * ny + ty) * R1);
+#pragma unroll
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int q = ((dz + 2) * 5 + dy + 2) * 5 + dx + 2;
+                const WT h = WeightIO<WT>::store_value(NEAR_BIN(q) * nscale);
+                const float wq = WeightIO<WT>::to_float(h);
+                if (wq > 0.0f) wacc += wq;
+                gstore(wp + (size_t)q * ri.len, h);
+                if (wq > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x + dx - ti.xa), h);
+                if (e.y >= 0) tmpl[e.y * kNQ + q] = wq;
+            }
+        }
+    gstore(V.W + v, wacc);
+    if (e.y >= 0) tmpl_sum[e.y] = wacc;
+}
+
+// Copy the templates to the sources with a uniform neighbourhood (cls >= 0) and write zeros for
+// inactive voxels of the row segments (cls = -2); listed sources (cls = -1) were written by
+// k_near_list. Same stores as k_near, values from the template of the source's density.
+template <class WT>
+__global__ __launch_bounds__(256) void k_near_fill(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                    int n_items, const signed char* __restrict__ cls,
+                                                    const float* __restrict__ tmpl,
+                                                    const float* __restrict__ tmpl_sum) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    if (it.w <= 0) return;
+    const Vol& V = vols[it.x];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const int c = valid ? (int)cls[loc] : -1;
+    const bool write = valid && c != -1;
+    const int R1 = V.n_rays + 1;
+    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
+    WT* wp = (WT*)V.wbuf + ri.off + (x0 - ri.xa);
+    WT* tb = (WT*)V.tbuf;
+    const float* tp = tmpl + (c >= 0 ? c : 0) * kNQ;
+#pragma unroll 5
+    for (int q = 0; q < kNQ; ++q) {
+        const float w = c >= 0 ? tp[q] : 0.0f;
+        const WT h = WeightIO<WT>::store_value(w);  // exact: templates hold stored values
+        if (write) gstore(wp + (size_t)q * ri.len + lane, h);
+        const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
+        if (__any(w > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
+            const RowInfo ti = gload(V.tinfo + ((size_t)tz * V.ny + ty) * R1);
+            if (w > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x0 + lane + (q % 5 - 2) - ti.xa), h);
+        }
+    }
+    if (c >= 0) gstore(V.W + loc, tmpl_sum[c]);
+}
+
+// Near-block memo, classification. A source is uniform when all 125 densities of its 5x5x5
+// block equal its own; a block reaching outside the grid includes the +inf halo or margin and
+// never is. Uniform densities get template ids from a 64-slot table (first come, first served:
+// the ids only name templates, the weights do not depend on them), and the representative of
+// each template is its lowest voxel index. cls: -2 inactive, -1 marched, t >= 0 template t.
+constexpr int kMaxTemplates = 64;
+constexpr unsigned kEmptyKey = 0xffffffffu;  // a NaN pattern; densities are finite
+
+__device__ inline int wave_min_i(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_classify(const Vol* __restrict__ vols, signed char* __restrict__ cls,
+                                                   unsigned* __restrict__ keys, int* __restrict__ rep) {
+    const Vol& V = vols[0];
+    const int nx = V.nx, ny = V.ny;
+    const int n = nx * ny * V.nz;
+    const long long vl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool in = vl < n;
+    const int v = in ? (int)vl : n - 1, vv = v;
+    const int x = vv % nx, row = vv / nx, y = row % ny, z = row / ny;
+    const bool act = in && V.support[vv];
+    const long long sy = nx + 2, sz = (long long)(ny + 2) * (nx + 2);
+    const float* c = V.rho_p + (z + 1) * sz + (y + 1) * sy + x + 1;
+    const float r0 = *c;
+    bool uni = act;
+    for (int dz = -2; dz <= 2 && uni; ++dz)
+        for (int dy = -2; dy <= 2 && uni; ++dy) {
+            const float* rr = c + dz * sz + dy * sy;
+            uni = rr[-2] == r0 && rr[-1] == r0 && rr[0] == r0 && rr[1] == r0 && rr[2] == r0;
+        }
+    const unsigned bits = __float_as_uint(r0);
+    int id = -1;
+    bool need = uni;
+    unsigned long long pending = __ballot(need);
+    while (pending) {  // one distinct density of the wave per pass
+        const int leader = __ffsll((long long)pending) - 1;
+        const unsigned key = __shfl(bits, leader);
+        int slot = -1;
+        if (lane == leader)
+            for (int k = 0; k < kMaxTemplates; ++k) {
+                unsigned cur = __hip_atomic_load(keys + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cur == kEmptyKey) cur = atomicCAS(keys + k, kEmptyKey, key), cur = cur == kEmptyKey ? key : cur;
+                if (cur == key) { slot = k; break; }
+            }
+        slot = __shfl(slot, leader);
+        const bool mine = need && bits == key;
+        if (slot >= 0) {
+            const int m = wave_min_i(mine ? v : INT_MAX);
+            if (lane == leader && m < __hip_atomic_load(rep + slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(rep + slot, m);
+        }
+        if (mine) id = slot, need = false;
+        pending = __ballot(need);
+    }
+    if (in) cls[v] = act ? (signed char)(uni ? id : -1) : (signed char)-2;
+}
+
+// The march list of the memo: marched sources and template representatives, appended per wave
+// (the order only changes which lanes march together). Also counts the uniform sources.
+__global__ __launch_bounds__(256) void k_memo_list(int n, const signed char* __restrict__ cls,
+                                                    const int* __restrict__ rep, int2* __restrict__ list,
+                                                    int* __restrict__ ctr) {
+    const long long vl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int v = (int)min(vl, (long long)n - 1);
+    const int c = vl < n ? (int)cls[v] : -2;
+    const bool take = c == -1 || (c >= 0 && rep[c] == v);
+    const unsigned long long b = __ballot(take), u = __ballot(c >= 0);
+    int base = 0;
+    if (lane == 0) {
+        if (b) base = atomicAdd(ctr, __popcll(b));
+        if (u) atomicAdd(ctr + 1, __popcll(u));
+    }
+    base = __shfl(base, 0);
+    if (take) list[base + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
 }
 
 // Far rays. PASS 1 counts the stored steps of every (row, ray); PASS 2 writes the weights and
@@ -1128,6 +1317,15 @@ struct prc_projector {
     std::vector<int4> h_items_act, h_items_all;
     std::vector<unsigned char> h_support;
     unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
+    // Near-block memoisation (uniform 5x5x5 neighbourhoods): per-voxel class (-2 inactive,
+    // -1 marched, t >= 0 template t), the marched list, and the templates.
+    bool memo = false;
+    signed char* cls = nullptr;
+    int2* near_list = nullptr;
+    int n_list = 0;
+    float* tmpl = nullptr;
+    float* tmpl_sum = nullptr;
+    int* memo_meta = nullptr;  // template keys [64], representatives [64], list and uniform counts
     float* scratch_in = nullptr;   // margin-padded input of the device-pointer forward
     float* scratch_adj = nullptr;  // margin-padded input of the device-pointer adjoint
     prc_stats stats{};
@@ -1211,8 +1409,16 @@ template <class WT>
 void launch_near(prc_projector* A) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
     ProfScope ps("near", 0);
-    if (A->hv.exact) k_near<true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
-    else k_near<false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+    if (A->memo) {  // march the listed sources, then copy the templates to the uniform ones
+        const unsigned nl = blocks_for(A->n_list, 256);
+        if (A->hv.exact) k_near_list<true, WT><<<nl, 256>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum, A->counters);
+        else k_near_list<false, WT><<<nl, 256>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum, A->counters);
+        k_near_fill<WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
+    } else if (A->hv.exact) {
+        k_near<true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+    } else {
+        k_near<false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -1249,6 +1455,13 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
             }
         }
     A->stats.n_active = n_active;
+    if (!(g.flags & PRC_FLAG_NO_NEAR_MEMO)) {
+        A->cls = dalloc<signed char>(n);
+        A->near_list = dalloc<int2>(std::max<long long>(n_active, 1));
+        A->tmpl = dalloc<float>((size_t)kMaxTemplates * kNQ);
+        A->tmpl_sum = dalloc<float>(kMaxTemplates);
+        A->memo_meta = dalloc<int>(2 * kMaxTemplates + 2);
+    }
     A->rho_alloc = dalloc<float>(np + 2 * margin);
     HIPCHK(hipMemcpy(A->rho_alloc, rp.data(), (np + 2 * margin) * sizeof(float), hipMemcpyHostToDevice));
     A->rho_p = A->rho_alloc + margin;
@@ -1338,6 +1551,17 @@ void normalise_projector(prc_projector* A) {
     HIPCHK(hipMemsetAsync(A->rinfo, 0, rows * R1 * sizeof(RowInfo), 0));
     HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), 0));
     HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), 0));
+    if (A->cls) {
+        ProfScope ps("near_classify", 0);
+        int* keys = A->memo_meta;
+        HIPCHK(hipMemsetAsync(keys, 0xff, kMaxTemplates * sizeof(int), 0));
+        HIPCHK(hipMemsetAsync(keys + kMaxTemplates, 0x7f, kMaxTemplates * sizeof(int), 0));
+        HIPCHK(hipMemsetAsync(keys + 2 * kMaxTemplates, 0, 2 * sizeof(int), 0));
+        const unsigned nb = blocks_for((long long)A->n, 256);
+        k_classify<<<nb, 256>>>(A->dv, A->cls, (unsigned*)keys, keys + kMaxTemplates);
+        k_memo_list<<<nb, 256>>>((int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
+        HIPCHK(hipGetLastError());
+    }
     if (A->wbytes == 2) launch_far<__half>(A, 1);
     else launch_far<float>(A, 1);
     {
@@ -1356,6 +1580,15 @@ void normalise_projector(prc_projector* A) {
     HIPCHK(hipMemcpy(total, A->total, sizeof(total), hipMemcpyDeviceToHost));
     std::vector<int> kglob(R1);
     HIPCHK(hipMemcpy(kglob.data(), A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost));
+    if (A->cls) {
+        int ctr[2];
+        HIPCHK(hipMemcpy(ctr, A->memo_meta + 2 * kMaxTemplates, sizeof(ctr), hipMemcpyDeviceToHost));
+        A->n_list = ctr[0];
+        A->stats.near_uniform = ctr[1];
+        // Worth a second pass only when a good share of the sources is uniform.
+        A->memo = ctr[1] > 0 && (long long)ctr[1] * 10 >= A->stats.n_active;
+        if (!A->memo) A->stats.near_uniform = 0;
+    }
     // Every stored step moves at most maxk voxels along each axis from its source.
     const int maxk = *std::max_element(kglob.begin(), kglob.end());
     const long long margin = (long long)(maxk + 1) * ((long long)V.nx * V.ny + V.nx + 1) + kWave;
@@ -1436,7 +1669,7 @@ void release(prc_projector* A) {
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
-    dfree(A->scratch_adj);
+    dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
     if (A->traw) (void)hipFree(A->traw);
     A->wraw = A->traw = A->wbuf = A->tbuf = nullptr;

@@ -178,3 +178,48 @@ def test_profiled_run_matches_graph_run(prc, phantom):
     np.testing.assert_array_equal(out, ref)
     assert rep["forward"]["count"] == 5 and rep["adjoint"]["count"] == 5
     assert all(v["mean_ms"] >= 0 for v in rep.values())
+
+
+@pytest.fixture(scope="module")
+def blocks():
+    """Piecewise-constant 24x20x18 phantom (lung, soft tissue and bone slabs, an air corner and
+    a sub-threshold pocket): most sources see one density over their 5x5x5 block."""
+    rho = np.full((18, 20, 24), 1.0, np.float32)
+    rho[:, :, :7] = 0.3
+    rho[:, 13:, 15:] = 1.85
+    rho[:4, :5, :5] = 1e-3
+    rho[9:12, 8:11, 10:13] = 0.02
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 2, rho.shape).astype(np.float32)
+    y = rng.uniform(0, 2, rho.shape).astype(np.float32)
+    return rho, x, y
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_uniform_neighbourhood_memo_is_bit_identical(prc, blocks, precision):
+    rho, x, y = blocks
+    announce(f"near memo vs full march 24x20x18 {precision}")
+    A = prc.Projector(rho, SPACING, "68Ga", 32, precision=precision)
+    B = prc.Projector(rho, SPACING, "68Ga", 32, precision=precision, near_memo=False)
+    sa, sb = A.stats, B.stats
+    assert sa["near_uniform"] > sa["n_active"] // 4 and sb["near_uniform"] == 0
+    assert sa["vrs_near"] < sb["vrs_near"] and sa["vrs_far"] == sb["vrs_far"]
+    np.testing.assert_array_equal(A.W, B.W)
+    np.testing.assert_array_equal(A.forward(x), B.forward(x))
+    np.testing.assert_array_equal(A.adjoint(y), B.adjoint(y))
+    A.close()
+    B.close()
+
+
+def test_uniform_neighbourhood_memo_matches_oracle(prc, oracle_mod, blocks):
+    rho, x, y = blocks
+    A = prc.Projector(rho, SPACING, "68Ga", 32)
+    table, _ = A.table
+    rho_p = np.maximum(rho, np.float32(1e-3))
+    O = oracle_mod.Projector(rho_p, SPACING, "68Ga", 32, table=table)
+    w_ref = np.zeros(rho.size)
+    w_ref[O.active] = O.W
+    np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
+    assert rel_l2(A.forward(x), O.forward(x)) <= 1e-6
+    assert rel_l2(A.adjoint(y), O.adjoint(y)) <= 1e-6
+    A.close()
