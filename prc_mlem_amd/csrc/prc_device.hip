@@ -45,7 +45,7 @@ constexpr int kNearMax = 8;              // a near ray visits at most 7 block vo
 constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
 constexpr int kWave = 64;
 constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
-constexpr int kFarGroup = 4;             // far-march steps whose density loads are issued together
+constexpr int kFarGroup = 8;             // far-march steps whose density loads are issued together
 
 // ---------------------------------------------------------------------------------------
 // Isotope model (prc.cu:38-57): Paneque-Yunta et al., Phys. Med. Biol. 70 (2025) 165013.
@@ -117,9 +117,11 @@ struct RowInfo {        // one per (row, ray); 32 bytes, one scalar load
     int xa, len;        // the row's active x range [xa, xa + len) (len 0: no active voxel)
     int pad0, pad1;
 };
-struct FarStep {        // march table of a far ray
+struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
     int dp;             // padded linear offset of the step's voxel from the source
+    short dx, dy, dz;   // the step's voxel relative to the source (the deposit's target)
+    short pad;
 };
 struct NearStep {
     float seg;
@@ -320,7 +322,7 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirs
             pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
             plin[(size_t)(1 + id) * V.LP + s] = linear_offset(V, i, j, k);
             const float seg = w.step();
-            fstep[(size_t)id * V.L + s] = FarStep{seg, (k * nyp + j) * nxp + i};
+            fstep[(size_t)id * V.L + s] = FarStep{seg, (k * nyp + j) * nxp + i, (short)i, (short)j, (short)k, 0};
         }
         kfirst[1 + id] = kf < 0 ? V.L : kf;
     } else if (id < V.n_rays + kNearRays) {
@@ -355,27 +357,36 @@ struct Lane {
     bool alive;
 };
 
-// One DDA step of prc.cu:146-163 for the current lane; returns the deposit (0 if none).
+// March parameters of a volume, copied out of the descriptor once per wave: the kernels store
+// through pointers the compiler cannot tell apart from the descriptor, which would otherwise
+// reload these in every step.
+struct MarchConst {
+    float cutoff, scale, support_rho, air_cap;
+    int nb;
+    Branch br[3];
+    __device__ explicit MarchConst(const Vol& V)
+        : cutoff(V.cutoff), scale(V.scale), support_rho(V.support_rho), air_cap(V.air_cap), nb(V.nb),
+          br{V.br[0], V.br[1], V.br[2]} {}
+};
+
+// One DDA step of prc.cu:146-163 for the current lane; returns the deposit (0 if none). Written
+// with selects instead of branches (a dead lane computes and discards): the ray leaves the grid
+// at the +inf halo, dies in air after air_cap mm, and dies when the range reaches the cutoff.
 template <bool EXACT>
-__device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_here, const Vol& V,
-                                   const float* tab, unsigned long long& visits) {
-    float dep = 0.0f;
-    if (!(rho < INFINITY)) {  // the halo: the ray has left the grid (prc.cu:163)
-        st.alive = false;
-        return 0.0f;
-    }
-    ++visits;
+__device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_here, const MarchConst& V,
+                                   const float* tab, unsigned& visits) {
+    const bool inside = st.alive && rho < INFINITY;
+    visits += inside ? 1u : 0u;
     const float wn = fminf(fmaf(rho, seg, st.w), V.cutoff);
     const float sn = EXACT ? survival(V.br, V.nb, wn) : lookup(tab, V.scale, wn);
-    if (rho >= V.support_rho) {
-        st.air = 0.0f;
-        if (deposit_here && st.surv > sn) dep = st.surv - sn;
-    } else if ((st.air += seg) > V.air_cap) {
-        st.alive = false;
-        return 0.0f;
-    }
-    if (wn >= V.cutoff) st.alive = false;
-    st.w = wn, st.surv = sn;
+    const bool tissue = rho >= V.support_rho;
+    const float air = tissue ? 0.0f : st.air + seg;
+    const bool live = inside && (tissue || !(air > V.air_cap));
+    const float dep = (live && tissue && deposit_here && st.surv > sn) ? st.surv - sn : 0.0f;
+    st.w = live ? wn : st.w;
+    st.surv = live ? sn : st.surv;
+    st.air = live ? air : st.air;
+    st.alive = live && !(wn >= V.cutoff);
     return dep;
 }
 
@@ -394,7 +405,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
                                                               float& a0, float (&a1)[6], float (&a2)[18],
                                                               float (&a3)[32], float (&a4l)[18], float (&a4h)[18],
                                                               float (&a5)[24], float (&a6)[8],
-                                                              unsigned long long& visits) {
+                                                              unsigned& visits) {
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
         if (i < 6) a1[i] = 0.0f;
@@ -403,6 +414,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
         if (i < 24) a5[i] = 0.0f;
         if (i < 8) a6[i] = 0.0f;
     }
+    const MarchConst M(V);
     for (int r = 0; r < kNearRays; ++r) {
         const int len = gload(V.nlen + r);
         const NearStep* nsr = V.nstep + r * kNearMax;
@@ -416,9 +428,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
         for (int s = 0; s < 7; ++s) rv[s] = (s < len && act) ? rho_at(ns[s].dp) : INFINITY;
         Lane st;
         st.alive = act;
-        auto step = [&](int s) {
-            return st.alive ? march_step<EXACT>(st, rv[s], ns[s].seg, true, V, tab, visits) : 0.0f;
-        };
+        auto step = [&](int s) { return march_step<EXACT>(st, rv[s], ns[s].seg, true, M, tab, visits); };
         a0 += step(0);  // every near ray starts inside the source voxel
         if (len > 1) { const float d = step(1); a1[ns[1].loc] += d; }
         if (len > 2) { const float d = step(2); a2[ns[2].loc] += d; }
@@ -462,7 +472,7 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
     const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
-    unsigned long long visits = 0;
+    unsigned visits = 0;
     NEAR_ACCUMULATORS;
     near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rho0 + dp + lane); }, NEAR_ARGS, visits);
     // Near weights on the scale of one far ray (prc.cu:205-218); W starts with their sum.
@@ -510,7 +520,7 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
     const int nx = V.nx, ny = V.ny;
     const int v = e.x, x = v % nx, row = v / nx, y = row % ny, z = row / ny;
     const float* rb = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + x + 1;
-    unsigned long long visits = 0;
+    unsigned visits = 0;
     NEAR_ACCUMULATORS;
     near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rb + dp); }, NEAR_ARGS, visits);
     if (vrs) {
@@ -665,8 +675,21 @@ __global__ __launch_bounds__(256) void k_memo_list(int n, const signed char* __r
     if (take) list[base + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
 }
 
+__device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline float lane_of(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline long long lane_of(long long v, int l) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | lo);
+}
+
 // Far rays. PASS 1 counts the stored steps of every (row, ray); PASS 2 writes the weights and
-// finishes W (continuing from the near sum, in the reference's accumulation order).
+// finishes W (continuing from the near sum, in the reference's accumulation order). The step
+// table is read 64 steps at a time, one step per lane, and each step's values are broadcast
+// from its lane; in PASS 2 the same lane also resolves the step's target row (the target-major
+// plane offset), so a step costs one density load and the two weight stores.
 template <int PASS, bool EXACT, class WT>
 __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                               int n_items, unsigned long long* vrs, unsigned long long* ndep) {
@@ -676,75 +699,110 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     const Vol& V = vols[it.x];
     if (!EXACT) load_table(tab, V);
     if (item_id >= n_items || it.w <= 0) return;
+    const MarchConst M(V);
+    const int ny = V.ny, nz = V.nz, L = V.L, n_rays = V.n_rays, R1 = n_rays + 1;
+    const FarStep* fstep = V.fstep;
+    const int* kfirst = V.kfirst;
+    const RowInfo* rinfo = V.rinfo;
+    const RowInfo* tinfo = V.tinfo;
+    WT* wbuf = (WT*)V.wbuf;
+    WT* tbuf = (WT*)V.tbuf;
     const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, y = row % V.ny, z = row / V.ny, x0 = it.z;
+    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
     const bool valid = lane < it.w;
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
-    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
-    const int R1 = V.n_rays + 1;
+    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane;
     const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
-    unsigned long long visits = 0, deposits = 0;
+    unsigned visits = 0, deposits = 0;
     float wacc = (PASS == 2 && act) ? gload(V.W + loc) : 0.0f;
 
-    for (int r = 0; r < V.n_rays; ++r) {
-        const int kf = gload(V.kfirst + 1 + r);
-        const FarStep* fs = V.fstep + (size_t)r * V.L;
+    for (int r = 0; r < n_rays; ++r) {
+        const int kf = gload(kfirst + 1 + r);
+        const FarStep* fs = fstep + (size_t)r * L;
         int cnt = 0;
         WT* wp = nullptr;
         if (PASS == 2) {
-            const RowInfo ri = gload(V.rinfo + (size_t)row * R1 + 1 + r);
+            const RowInfo ri = gload(rinfo + (size_t)row * R1 + 1 + r);
             cnt = ri.cnt;
-            wp = (WT*)V.wbuf + ri.off + (x0 - xa) - (size_t)kf * len;
+            if (cnt <= 0) continue;  // no source of this row deposits along this ray
+            wp = wbuf + ri.off + (x0 - xa) - (size_t)kf * len + lane;
         }
+        // PASS 2 marches only up to the row's last stored step.
+        const int send = PASS == 2 ? kf + cnt : L;
         Lane st;
         st.alive = act;
         int last = -1;
         int s = 0;
-        for (; s < V.L; s += kFarGroup) {
-            // Density loads of the whole group are issued before any step is evaluated; a lane
-            // alive at the group start is at most kFarGroup voxels outside the grid by its end,
-            // inside the +inf margin of the padded density.
-            FarStep f[kFarGroup];
-            float rv[kFarGroup];
-            const bool go = st.alive;
-#pragma unroll
-            for (int u = 0; u < kFarGroup; ++u) f[u] = s + u < V.L ? gload(fs + s + u) : FarStep{0.0f, 0};
-#pragma unroll
-            for (int u = 0; u < kFarGroup; ++u) rv[u] = go ? gload(rho0 + f[u].dp + lane) : INFINITY;
-#pragma unroll
-            for (int u = 0; u < kFarGroup; ++u) {
-                const int su = s + u;
-                if (su >= V.L) break;
-                float dep = 0.0f;
-                if (st.alive) dep = march_step<EXACT>(st, rv[u], f[u].seg, su >= kf, V, tab, visits);
-                if (PASS == 1) {
-                    if (dep > 0.0f) last = su;
-                } else if (su >= kf && su - kf < cnt) {
-                    const WT h = WeightIO<WT>::store_value(dep);
-                    const float wv = WeightIO<WT>::to_float(h);
-                    if (wv > 0.0f) wacc += wv, ++deposits;
-                    if (valid) gstore(wp + (size_t)su * len + lane, h);
-                    // Target-major copy: step su of this source is step su of target s + D.
-                    if (__any(wv > 0.0f)) {
-                        const int4 d = gload(V.pstep + (size_t)(1 + r) * V.LP + su);
-                        const RowInfo ti = gload(V.tinfo + ((size_t)(z + d.z) * V.ny + (y + d.y)) * R1 + 1 + r);
-                        if (wv > 0.0f)
-                            gstore((WT*)V.tbuf + ti.off + (size_t)(su - kf) * ti.len + (x0 + lane + d.x - ti.xa), h);
-                    }
+        bool going = __any(act);
+        for (int s0 = 0; s0 < send && going; s0 += kWave) {
+            const int sl = s0 + lane;
+            const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, 0, 0, 0, 0, 0};
+            long long tl = 0;  // PASS 2: target-major element offset of step sl for lane 0
+            if (PASS == 2 && sl >= kf && sl < send) {
+                const int ty = y + fl.dy, tz = z + fl.dz;
+                if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)nz) {
+                    const RowInfo ti = gload(tinfo + ((size_t)tz * ny + ty) * R1 + 1 + r);
+                    tl = ti.off + (long long)(sl - kf) * ti.len + (x0 + fl.dx - ti.xa);
                 }
             }
-            if (!__any(st.alive)) {
-                s += kFarGroup;
-                break;
+            const int nst = min(kWave, send - s0);
+            for (int j = 0; j < nst; j += kFarGroup) {
+                // Density loads of the group first. A lane alive at the group start is inside the
+                // grid, so at most kFarGroup voxels outside it by the group's end: inside the +inf
+                // margin of the padded density.
+                // Every lane loads (a dead lane its own voxel), so the loads issue back to back.
+                float seg[kFarGroup], rv[kFarGroup];
+#pragma unroll
+                for (int u = 0; u < kFarGroup; ++u) {
+                    seg[u] = lane_of(fl.seg, j + u);
+                    const float v = gload(rho0 + (st.alive ? lane_of(fl.dp, j + u) : 0));
+                    rv[u] = st.alive ? v : INFINITY;
+                }
+                // Steps past nst are marched too, without stores: past the row's last stored step
+                // no lane of the row deposits (that is what the count pass measured), and past the
+                // table's end every lane is dead (a ray leaves the grid within nx+ny+nz steps).
+                WT hv[kFarGroup];
+#pragma unroll
+                for (int u = 0; u < kFarGroup; ++u) {
+                    const int su = s0 + j + u;
+                    const float dep = march_step<EXACT>(st, rv[u], seg[u], su >= kf, M, tab, visits);
+                    if (PASS == 1) {
+                        if (dep > 0.0f) last = su;
+                    } else {
+                        hv[u] = WeightIO<WT>::store_value(dep);
+                        const float wv = WeightIO<WT>::to_float(hv[u]);
+                        if (wv > 0.0f) wacc += wv, ++deposits;
+                    }
+                }
+                // The group's stores go out after its last density wait, so no wait of this
+                // group waits on a store.
+                if (PASS == 2) {
+#pragma unroll
+                    for (int u = 0; u < kFarGroup; ++u) {
+                        const int su = s0 + j + u;
+                        if (j + u >= nst) break;
+                        if (su < kf) continue;
+                        if (valid) gstore(wp + (size_t)su * len, hv[u]);
+                        // Target-major copy: step su of this source is step su of its target. A
+                        // deposit implies an active target inside its row's active range.
+                        const long long tb = lane_of(tl, j + u);
+                        if (WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + tb + lane, hv[u]);
+                    }
+                }
+                s = s0 + j + kFarGroup;
+                if (!__any(st.alive)) {
+                    going = false;
+                    break;
+                }
             }
         }
         if (PASS == 1) {
             const int kmax = wave_max(last + 1);
             if (lane == 0 && kmax > kf) atomicMax(&V.rinfo[(size_t)row * R1 + 1 + r].cnt, kmax - kf);
         } else {
-            for (s = max(s, kf); s < kf + cnt; ++s)
-                if (valid) gstore(wp + (size_t)s * len + lane, WeightIO<WT>::store_value(0.0f));
+            for (s = max(s, kf); s < send; ++s)
+                if (valid) gstore(wp + (size_t)s * len, WeightIO<WT>::store_value(0.0f));
         }
     }
     if (PASS == 2 && act) gstore(V.W + loc, wacc);
