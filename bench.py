@@ -51,6 +51,21 @@ def timed(fn, steps, warmup, sync):
     return time.perf_counter() - t0
 
 
+def measured_traffic(kernel, form):
+    """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
+    plain copy of known size), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_thorax_128x128x159.md")
+    try:
+        with open(path) as fh:
+            text = fh.read()
+        table = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+    except (OSError, ValueError, IndexError):
+        return None
+    entry = table.get("kernels", {}).get(f"{kernel}/{form}")
+    return None if entry is None else entry.get("hbm_bytes_per_launch")
+
+
 def cpu_baseline(rho, data, slices, seconds_budget):
     """The FP32 oracle (reference semantics, OpenMP) on a Z-slab of the workload: build +
     6 iterations. Returns the measured slab time, its voxel-ray-step rate and threads."""
@@ -74,9 +89,22 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4, help="volumes per batched step")
     ap.add_argument("--only", choices=["both", "mixed", "fp32"], default="both")
-    ap.add_argument("--cpu-slices", type=int, default=8, help="Z slices of the CPU-baseline slab (0 = skip)")
+    ap.add_argument("--cpu-slices", type=int, default=48, help="Z slices of the CPU-baseline slab (0 = skip)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes three times (the known-size kernel that "
+                         "scales the HBM byte counters of a profiler run) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 * n, "reads_per_launch": 4 * n, "writes_per_launch": 4 * n}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} (WORLD_SIZE {world}) is not supported; the "
@@ -186,6 +214,26 @@ def main():
             roofline = {"kernel": name, "bound": "valu", "achieved": None, "peak": None, "unit": None,
                         "frac": None, "traffic": None, "mean_ms": k[name]["mean_ms"]}
 
+    # Projector throughput in the survey's effective accounting: one call = the far voxel-ray
+    # steps plus 125 near taps per active source, at 8 B (FP32) or 6 B (mixed) per step. These
+    # are effective bytes, not HBM traffic (the stored-weight stream is the roofline above).
+    projector = {}
+    for form, d, vols, b_step in (("fp32", fp32, 1, 8), ("mixed", mixed, args.batch, 6)):
+        if d is None or "kernels" not in d:
+            continue
+        n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
+        vrs_call = d["vrs_far"] + 125 * n_act
+        entry = {"vrs_per_call": vrs_call, "bytes_per_step": b_step}
+        for op in ("forward", "adjoint"):
+            if op in d["kernels"]:
+                t = d["kernels"][op]["mean_ms"] / 1e3 / vols  # per volume
+                entry[op] = {"ms_per_volume": round(t * 1e3, 3), "vrs_per_s": round(vrs_call / t, 1),
+                             "effective_GBps": round(vrs_call * b_step / t / 1e9, 1),
+                             "effective_frac_of_hbm_peak": round(vrs_call * b_step / t / 1e9 / HBM_PEAK_GBS, 3)}
+        projector[form] = entry
+    if roofline is not None:
+        roofline["traffic"] = measured_traffic(roofline["kernel"], "mixed" if mixed is not None else "fp32")
+
     cpu = None
     if args.cpu_slices > 0 and fp32 is not None:
         log(f"cpu baseline: oracle on a {args.cpu_slices}-slice slab")
@@ -213,7 +261,7 @@ def main():
                    "batch": args.batch if mixed is not None else 1, "form": "mixed+batched" if mixed else "fp32"},
         "fp32_vols_per_s": round(fp32["vols_per_s"], 4) if fp32 else None,
         "speedup_mixed_batched_vs_fp32": round(mixed["vols_per_s"] / fp32["vols_per_s"], 3) if (mixed and fp32) else None,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "projector_effective": projector,
     }
     out.update(result)
     if fp32 is not None:

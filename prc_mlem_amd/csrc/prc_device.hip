@@ -949,10 +949,17 @@ __device__ inline float forward_scale(float f, float w) {
 
 // Eight step offsets of one ray in a single scalar load (the offset table carries 8 entries of
 // slack past its end); entries past `count` are replaced by 0 so every gather stays in bounds.
-__device__ inline void load_offsets8(const int* pl, int j, int count, int (&dl)[8]) {
-    const v8i d = *(const GLOBAL_AS v8i*)(pl + j);
+// Steps per group of the projector passes: their weight and gather loads are all issued
+// before the first is used.
+constexpr int kProjGroup = 16;
+
+__device__ inline void load_offsets(const int* pl, int j, int count, int (&dl)[kProjGroup]) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) dl[u] = j + u < count ? d[u] : 0;
+    for (int h = 0; h < kProjGroup; h += 8) {
+        const v8i d = *(const GLOBAL_AS v8i*)(pl + j + h);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) dl[h + u] = j + h + u < count ? d[u] : 0;
+    }
 }
 
 // Shared body of both operators: acc += sum over the stored steps of one ray of
@@ -961,19 +968,19 @@ __device__ inline void load_offsets8(const int* pl, int j, int count, int (&dl)[
 template <class WT>
 __device__ inline float ray_planes(const WT* wrow, int len, int count, const int* pl, const float* vec0, int sign,
                                    int lane, float acc) {
-    for (int j = 0; j < count; j += 8) {
-        int dl[8];
-        load_offsets8(pl, j, count, dl);
-        float w[8], v[8];
+    for (int j = 0; j < count; j += kProjGroup) {
+        int dl[kProjGroup];
+        load_offsets(pl, j, count, dl);
+        float w[kProjGroup], v[kProjGroup];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kProjGroup; ++u) {
             const WT* wp = wrow + min(j + u, count - 1) * len;  // uniform plane base
             const float* vp = vec0 + sign * dl[u];              // uniform gather base
             w[u] = WeightIO<WT>::load(wp + lane);
             v[u] = gload(vp + lane);
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < kProjGroup; ++u) {
             const bool m = (j + u < count) & (w[u] != 0.0f);
             acc = m ? fmaf(w[u], v[u], acc) : acc;
         }
@@ -1547,8 +1554,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LP = std::max(L, kNQ);
     A->fstep = dalloc<FarStep>((size_t)R * L);
     A->pstep = dalloc<int4>((size_t)R1 * LP);
-    A->plin = dalloc<int>((size_t)R1 * LP + 8);  // 8 entries of slack for 8-wide scalar loads
-    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + 8) * sizeof(int)));
+    A->plin = dalloc<int>((size_t)R1 * LP + 16);  // slack for the group-wide scalar loads
+    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + 16) * sizeof(int)));
     A->kfirst = dalloc<int>(R1);
     A->kglob = dalloc<int>(R1);
     A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
@@ -1678,7 +1685,7 @@ void audit_layout(prc_projector* A) {
     const size_t rows = (size_t)V.ny * V.nz;
     const int R1 = V.n_rays + 1, LP = V.LP;
     std::vector<RowInfo> ri(rows * R1), ti(rows * R1);
-    std::vector<int> kf(R1), pl((size_t)R1 * LP + 8);
+    std::vector<int> kf(R1), pl((size_t)R1 * LP + 16);
     HIPCHK(hipMemcpy(ri.data(), A->rinfo, ri.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ti.data(), A->tinfo, ti.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(kf.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
@@ -1693,7 +1700,7 @@ void audit_layout(prc_projector* A) {
             if (lo < -kWave || hi >= total + kWave)
                 fail(PRC_ERR_INVALID, "audit: %s planes of row %d ray %d read [%lld, %lld] outside [%d, %lld)", what,
                      row, p, lo, hi, -kWave, total + kWave);
-            const long long last = (long long)p * LP + kf[p] + ((e.cnt - 1) / 8) * 8 + 7;
+            const long long last = (long long)p * LP + kf[p] + ((e.cnt - 1) / kProjGroup) * kProjGroup + kProjGroup - 1;
             if (kf[p] + e.cnt > LP || last >= (long long)pl.size())
                 fail(PRC_ERR_INVALID, "audit: %s offsets of ray %d read past the table", what, p);
             for (int j = 0; j < e.cnt; ++j) {
