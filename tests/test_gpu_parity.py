@@ -132,9 +132,13 @@ def test_batch_equals_single_runs(prc, phantom):
     rho2 = np.ascontiguousarray(rho[:, ::-1, :])
     A2 = prc.Projector(rho2, (0.9, 1.2, 1.5), "18F", 16)
     x2 = np.ascontiguousarray(x[:, ::-1, :])
+    rho3 = np.ascontiguousarray(rho[1:, 2:, 3:])  # uneven grid: 11 x 12 x 13
+    A3 = prc.Projector(rho3, (1.3, 0.8, 1.0), "11C", 24)
+    x3 = np.ascontiguousarray(x[1:, 2:, 3:])
     singles = [prc.correct(A1, x, 5, 0.1, 20.0, True), prc.correct(A2, x2, 5, 0.05, 20.0, True),
-               prc.correct(A1, y, 5, 0.0, 20.0, True)]
-    batch = prc.batch_correct([A1, A2, A1], [x, x2, y], 5, beta=[0.1, 0.05, 0.0], gamma=20.0, accelerate=True)
+               prc.correct(A1, y, 5, 0.0, 20.0, True), prc.correct(A3, x3, 5, 0.2, 20.0, True)]
+    batch = prc.batch_correct([A1, A2, A1, A3], [x, x2, y, x3], 5, beta=[0.1, 0.05, 0.0, 0.2], gamma=20.0,
+                              accelerate=True)
     for a, b in zip(batch, singles):
         np.testing.assert_array_equal(a, b)
 
