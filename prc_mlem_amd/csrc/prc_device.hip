@@ -110,11 +110,15 @@ float range_cutoff_host(const Isotope& m, float fraction) {
 
 // ---------------------------------------------------------------------------------------
 // Device data layout.
-struct RowInfo {        // one per (row, ray); 32 bytes, one scalar load
-    long long off;      // element offset of the row's planes for this ray in the weight buffer
-    int cnt;            // stored steps
-    int kt;             // forward bound: last step (exclusive) at which this row receives weight
-    int xa, len;        // the row's active x range [xa, xa + len) (len 0: no active voxel)
+// A row's active x range [xa, xa + len) is cut into segments of up to 64 voxels starting at xa:
+// segment s covers [xa + 64 s, xa + 64 s + min(64, len - 64 s)). One wave handles one segment,
+// and every (segment, ray) owns one contiguous run of planes, so a wave streams memory no other
+// wave touches. Unit index of segment s of row r: r * nseg + s (nseg = ceil(nx / 64)).
+struct RowInfo {        // one per (segment, ray); 32 bytes, one scalar load
+    long long off;      // element offset of this segment's planes for this ray in the weight buffer
+    int cnt;            // stored steps (planes)
+    int kt;             // forward bound: last step (exclusive) at which this segment receives weight
+    int xa, len;        // the segment's x range [xa, xa + len) = plane width (len 0: empty segment)
     int pad0, pad1;
 };
 struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
@@ -163,8 +167,9 @@ struct Vol {            // device descriptor of one volume
     float* W;           // per voxel
     const int* row_xa;  // [ny*nz]
     const int* row_len; // [ny*nz], 0 = no active voxel
-    RowInfo* rinfo;     // [ny*nz][n_rays + 1] source-major layout (planes over source x)
-    RowInfo* tinfo;     // [ny*nz][n_rays + 1] target-major layout (planes over target x)
+    int nseg;           // segment slots per row, ceil(nx / 64)
+    RowInfo* rinfo;     // [ny*nz*nseg][n_rays + 1] source-major layout (planes over source x)
+    RowInfo* tinfo;     // [ny*nz*nseg][n_rays + 1] target-major layout (planes over target x)
     void* wbuf;         // source-major weights (float or __half): the adjoint's stream
     void* tbuf;         // target-major weights: the forward's stream
     const FarStep* fstep;   // [n_rays][L]
@@ -395,6 +400,39 @@ __device__ inline void load_table(float* tab, const Vol& V) {
     __syncthreads();
 }
 
+// Target-major slots of a wave's deposits. The wave's lanes hit targets tx = tx0 + lane of row t,
+// which fall into at most two segments of that row; lanes below `thr` use the first. For plane k
+// the slot of a lane is tbuf[b + lane] with b from its segment; `ok` bit 0 / bit 1 says whether
+// the segment holds plane k (it always does where a deposit lands; the check keeps every store
+// inside its own run regardless). unit0 points at segment 0 of row t for the ray.
+struct TargetPair {
+    long long b0, b1;
+    int thr, ok;
+};
+__device__ inline TargetPair target_pair(const RowInfo* unit0, int R1, int xa_t, int len_t, int tx0, int k) {
+    TargetPair tp{0, 0, kWave, 0};
+    const int rel = tx0 - xa_t;
+    const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);  // floor
+    const int ns = (len_t + kWave - 1) / kWave;
+    tp.thr = kWave * (s0 + 1) - rel;  // 1 .. 64
+    if (s0 >= 0 && s0 < ns) {
+        const RowInfo e = gload(unit0 + (size_t)s0 * R1);
+        tp.ok |= k < e.cnt ? 1 : 0;
+        tp.b0 = e.off - e.xa + (long long)k * e.len + tx0;
+    }
+    if (s0 + 1 >= 0 && s0 + 1 < ns) {
+        const RowInfo e = gload(unit0 + (size_t)(s0 + 1) * R1);
+        tp.ok |= k < e.cnt ? 2 : 0;
+        tp.b1 = e.off - e.xa + (long long)k * e.len + tx0;
+    }
+    return tp;
+}
+__device__ inline bool target_slot(const TargetPair& tp, int lane, long long& idx) {
+    const bool hi = lane >= tp.thr;
+    idx = (hi ? tp.b1 : tp.b0) + lane;
+    return ((tp.ok >> (hi ? 1 : 0)) & 1) != 0;
+}
+
 // The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into the shell
 // registers (shell 4 split in two halves of 18 to stay within register indexing). rho_at(dp)
 // returns the density at padded offset dp from the lane's source.
@@ -476,9 +514,10 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     NEAR_ACCUMULATORS;
     near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rho0 + dp + lane); }, NEAR_ARGS, visits);
     // Near weights on the scale of one far ray (prc.cu:205-218); W starts with their sum.
-    const int R1 = V.n_rays + 1;
-    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
-    WT* wp = (WT*)V.wbuf + ri.off + (x0 - ri.xa);
+    const int R1 = V.n_rays + 1, nseg = V.nseg;
+    const int seg = (x0 - gload(V.row_xa + row)) / kWave;
+    const RowInfo ri = gload(V.rinfo + ((size_t)row * nseg + seg) * R1);
+    WT* wp = (WT*)V.wbuf + ri.off;
     WT* tb = (WT*)V.tbuf;
     const float nscale = (float)V.n_rays / (float)kNearRays;
     float wacc = 0.0f;
@@ -491,8 +530,11 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
         // A deposit implies an active target, inside its row's active range.
         const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
         if (__any(wq > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
-            const RowInfo ti = gload(V.tinfo + ((size_t)tz * V.ny + ty) * R1);
-            if (wq > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x0 + lane + (q % 5 - 2) - ti.xa), h);
+            const int t = tz * V.ny + ty;
+            const TargetPair tp = target_pair(V.tinfo + (size_t)t * nseg * R1, R1, gload(V.row_xa + t),
+                                              gload(V.row_len + t), x0 + (q % 5 - 2), q);
+            long long idx;
+            if (target_slot(tp, lane, idx) && wq > 0.0f) gstore(tb + idx, h);
         }
     }
     if (act) gstore(V.W + loc, wacc);
@@ -528,8 +570,8 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
         if ((threadIdx.x & (kWave - 1)) == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
     }
     if (!act) return;
-    const int R1 = V.n_rays + 1;
-    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
+    const int R1 = V.n_rays + 1, nseg = V.nseg;
+    const RowInfo ri = gload(V.rinfo + ((size_t)row * nseg + (x - gload(V.row_xa + row)) / kWave) * R1);
     WT* wp = (WT*)V.wbuf + ri.off + (x - ri.xa);
     WT* tb = (WT*)V.tbuf;
     const float nscale = (float)V.n_rays / (float)kNearRays;
@@ -538,9 +580,20 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
     for (int dz = -2; dz <= 2; ++dz)
 #pragma unroll
         for (int dy = -2; dy <= 2; ++dy) {
+            // The five targets x-2 .. x+2 of row t lie in at most two of its segments.
             const int ty = y + dy, tz = z + dz;
-            RowInfo ti{};
-            if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)V.nz) ti = gload(V.tinfo + ((size_t)tz * ny + ty) * R1);
+            RowInfo t0{}, t1{};
+            int txa = 0;
+            if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)V.nz) {
+                const int t = tz * ny + ty, tlen = gload(V.row_len + t);
+                txa = gload(V.row_xa + t);
+                const int rel = x - 2 - txa;
+                const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);
+                const RowInfo* u0 = V.tinfo + (size_t)t * nseg * R1;
+                const int ns = (tlen + kWave - 1) / kWave;
+                if (s0 >= 0 && s0 < ns) t0 = gload(u0 + (size_t)s0 * R1);
+                if (s0 + 1 >= 0 && s0 + 1 < ns) t1 = gload(u0 + (size_t)(s0 + 1) * R1);
+            }
 #pragma unroll
             for (int dx = -2; dx <= 2; ++dx) {
                 const int q = ((dz + 2) * 5 + dy + 2) * 5 + dx + 2;
@@ -548,7 +601,11 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
                 const float wq = WeightIO<WT>::to_float(h);
                 if (wq > 0.0f) wacc += wq;
                 gstore(wp + (size_t)q * ri.len, h);
-                if (wq > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x + dx - ti.xa), h);
+                // A deposit implies an active target: inside one of the two segments.
+                const int tx = x + dx;
+                const RowInfo& te = (t1.len > 0 && tx >= t1.xa) ? t1 : t0;
+                if (wq > 0.0f && tx >= te.xa && tx < te.xa + te.len && q < te.cnt)
+                    gstore(tb + te.off + (size_t)q * te.len + (tx - te.xa), h);
                 if (e.y >= 0) tmpl[e.y * kNQ + q] = wq;
             }
         }
@@ -575,20 +632,24 @@ __global__ __launch_bounds__(256) void k_near_fill(const Vol* __restrict__ vols,
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const int c = valid ? (int)cls[loc] : -1;
     const bool write = valid && c != -1;
-    const int R1 = V.n_rays + 1;
-    const RowInfo ri = gload(V.rinfo + (size_t)row * R1);
-    WT* wp = (WT*)V.wbuf + ri.off + (x0 - ri.xa);
+    const int R1 = V.n_rays + 1, nseg = V.nseg;
+    const int seg = (x0 - gload(V.row_xa + row)) / kWave;
+    const RowInfo ri = gload(V.rinfo + ((size_t)row * nseg + seg) * R1);
+    WT* wp = (WT*)V.wbuf + ri.off;
     WT* tb = (WT*)V.tbuf;
-    const float* tp = tmpl + (c >= 0 ? c : 0) * kNQ;
+    const float* tpl = tmpl + (c >= 0 ? c : 0) * kNQ;
 #pragma unroll 5
     for (int q = 0; q < kNQ; ++q) {
-        const float w = c >= 0 ? tp[q] : 0.0f;
+        const float w = c >= 0 ? tpl[q] : 0.0f;
         const WT h = WeightIO<WT>::store_value(w);  // exact: templates hold stored values
         if (write) gstore(wp + (size_t)q * ri.len + lane, h);
         const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
         if (__any(w > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
-            const RowInfo ti = gload(V.tinfo + ((size_t)tz * V.ny + ty) * R1);
-            if (w > 0.0f) gstore(tb + ti.off + (size_t)q * ti.len + (x0 + lane + (q % 5 - 2) - ti.xa), h);
+            const int t = tz * V.ny + ty;
+            const TargetPair tp = target_pair(V.tinfo + (size_t)t * nseg * R1, R1, gload(V.row_xa + t),
+                                              gload(V.row_len + t), x0 + (q % 5 - 2), q);
+            long long idx;
+            if (target_slot(tp, lane, idx) && w > 0.0f) gstore(tb + idx, h);
         }
     }
     if (c >= 0) gstore(V.W + loc, tmpl_sum[c]);
@@ -713,7 +774,10 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
     const float* rho0 = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane;
-    const int xa = gload(V.row_xa + row), len = gload(V.row_len + row);
+    const int nseg = V.nseg;
+    const size_t unit = (size_t)row * nseg + (x0 - gload(V.row_xa + row)) / kWave;  // this segment
+    const int* row_xa = V.row_xa;
+    const int* row_len = V.row_len;
     unsigned visits = 0, deposits = 0;
     float wacc = (PASS == 2 && act) ? gload(V.W + loc) : 0.0f;
 
@@ -723,11 +787,12 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
         int cnt = 0;
         WT* wp = nullptr;
         if (PASS == 2) {
-            const RowInfo ri = gload(rinfo + (size_t)row * R1 + 1 + r);
+            const RowInfo ri = gload(rinfo + unit * R1 + 1 + r);
             cnt = ri.cnt;
-            if (cnt <= 0) continue;  // no source of this row deposits along this ray
-            wp = wbuf + ri.off + (x0 - xa) - (size_t)kf * len + lane;
+            if (cnt <= 0) continue;  // no source of this segment deposits along this ray
+            wp = wbuf + ri.off - (long long)kf * ri.len + lane;
         }
+        const int len = PASS == 2 ? it.w : 0;  // plane width = the segment's width
         // PASS 2 marches only up to the row's last stored step.
         const int send = PASS == 2 ? kf + cnt : L;
         Lane st;
@@ -738,12 +803,14 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
         for (int s0 = 0; s0 < send && going; s0 += kWave) {
             const int sl = s0 + lane;
             const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, 0, 0, 0, 0, 0};
-            long long tl = 0;  // PASS 2: target-major element offset of step sl for lane 0
+            // PASS 2: target-major slots of step sl (its target row's one or two segments).
+            TargetPair tl{0, 0, kWave, 0};
             if (PASS == 2 && sl >= kf && sl < send) {
                 const int ty = y + fl.dy, tz = z + fl.dz;
                 if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)nz) {
-                    const RowInfo ti = gload(tinfo + ((size_t)tz * ny + ty) * R1 + 1 + r);
-                    tl = ti.off + (long long)(sl - kf) * ti.len + (x0 + fl.dx - ti.xa);
+                    const int t = tz * ny + ty;
+                    tl = target_pair(tinfo + (size_t)t * nseg * R1 + 1 + r, R1, gload(row_xa + t), gload(row_len + t),
+                                     x0 + fl.dx, sl - kf);
                 }
             }
             const int nst = min(kWave, send - s0);
@@ -786,8 +853,10 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
                         if (valid) gstore(wp + (size_t)su * len, hv[u]);
                         // Target-major copy: step su of this source is step su of its target. A
                         // deposit implies an active target inside its row's active range.
-                        const long long tb = lane_of(tl, j + u);
-                        if (WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + tb + lane, hv[u]);
+                        const TargetPair tp{lane_of(tl.b0, j + u), lane_of(tl.b1, j + u), lane_of(tl.thr, j + u),
+                                            lane_of(tl.ok, j + u)};
+                        long long idx;
+                        if (target_slot(tp, lane, idx) && WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + idx, hv[u]);
                     }
                 }
                 s = s0 + j + kFarGroup;
@@ -798,8 +867,9 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
             }
         }
         if (PASS == 1) {
+            // One wave per segment: its count is the segment's.
             const int kmax = wave_max(last + 1);
-            if (lane == 0 && kmax > kf) atomicMax(&V.rinfo[(size_t)row * R1 + 1 + r].cnt, kmax - kf);
+            if (lane == 0 && kmax > kf) V.rinfo[unit * R1 + 1 + r].cnt = kmax - kf;
         } else {
             for (s = max(s, kf); s < send; ++s)
                 if (valid) gstore(wp + (size_t)s * len, WeightIO<WT>::store_value(0.0f));
@@ -816,24 +886,30 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     }
 }
 
-// Plane offsets of the row-plane layout, computed on the device after the counting pass:
-// near block = 125 steps for every row holding active voxels, rows without any hold nothing.
-__global__ void k_row_totals(Vol V, long long* row_tot) {
-    const int rows = V.ny * V.nz, R1 = V.n_rays + 1;
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= rows) return;
-    const int len = gload(V.row_len + row), xa = gload(V.row_xa + row);
+// Plane offsets of the segment layout, computed on the device after the counting pass: the near
+// block = 125 planes for every non-empty segment; empty segments hold nothing.
+__device__ inline int segment_width(int len, int seg) { return min(kWave, max(0, len - kWave * seg)); }
+// Every (segment, ray) run starts on a 64-entry boundary, so a full segment's planes are whole
+// cache lines; only runs of partial segments are padded (at most 63 entries each).
+__host__ __device__ inline long long run_size(int cnt, int w) { return ((long long)cnt * w + kWave - 1) / kWave * kWave; }
+
+__global__ void k_row_totals(Vol V, long long* unit_tot) {
+    const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
+    const int unit = blockIdx.x * blockDim.x + threadIdx.x;
+    if (unit >= units) return;
+    const int row = unit / V.nseg, seg = unit % V.nseg;
+    const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
     long long off = 0;
     for (int p = 0; p < R1; ++p) {
-        RowInfo* e = V.rinfo + (size_t)row * R1 + p;
-        const int cnt = len == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
+        RowInfo* e = V.rinfo + (size_t)unit * R1 + p;
+        const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
         e->cnt = cnt;
         e->off = off;
-        e->xa = xa;
-        e->len = len;
-        off += (long long)cnt * len;
+        e->xa = x0;
+        e->len = w;
+        off += run_size(cnt, w);
     }
-    row_tot[row] = off;
+    unit_tot[unit] = off;
 }
 
 // Exclusive prefix sum of the row totals by one block in a fixed order; *total = grand total.
@@ -862,19 +938,19 @@ __global__ __launch_bounds__(1024) void k_scan_rows(const long long* __restrict_
     if (t == nt - 1) *total = part[nt - 1];
 }
 
-__global__ void k_row_offsets(Vol V, const long long* __restrict__ row_base) {
+__global__ void k_row_offsets(Vol V, const long long* __restrict__ unit_base) {
     const int R1 = V.n_rays + 1;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)V.ny * V.nz * R1) return;
-    V.rinfo[id].off += row_base[id / R1];
+    if (id >= (long long)V.ny * V.nz * V.nseg * R1) return;
+    V.rinfo[id].off += unit_base[id / R1];
 }
 
-// kglob[p] = kfirst[p] + max over rows of cnt(row, p): the forward's search bound per ray.
+// kglob[p] = kfirst[p] + max over segments of cnt(segment, p): the forward's search bound per ray.
 __global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
     __shared__ int sm[256];
-    const int p = blockIdx.x, R1 = V.n_rays + 1, rows = V.ny * V.nz;
+    const int p = blockIdx.x, R1 = V.n_rays + 1, units = V.ny * V.nz * V.nseg;
     int m = 0;
-    for (int row = threadIdx.x; row < rows; row += blockDim.x) m = max(m, V.rinfo[(size_t)row * R1 + p].cnt);
+    for (int u = threadIdx.x; u < units; u += blockDim.x) m = max(m, V.rinfo[(size_t)u * R1 + p].cnt);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -884,58 +960,66 @@ __global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
     if (threadIdx.x == 0) kglob[p] = sm[0] > 0 ? gload(V.kfirst + p) + sm[0] : 0;
 }
 
-// Forward bound per (row, ray): the last step at which some source row reaches this row.
+// Forward bound per (segment, ray): one past the last step at which some source segment that
+// stores that step reaches this segment's x range (an upper bound of where deposits land).
 __global__ void k_forward_bounds(Vol V, const int* kglob) {
-    const int R1 = V.n_rays + 1;
+    const int R1 = V.n_rays + 1, nseg = V.nseg;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)V.ny * V.nz * R1) return;
-    const int row = (int)(id / R1), p = (int)(id % R1);
-    const int len = gload(V.row_len + row);
+    if (id >= (long long)V.ny * V.nz * nseg * R1) return;
+    const int unit = (int)(id / R1), p = (int)(id % R1);
+    const int row = unit / nseg, seg = unit % nseg;
+    const int tw = segment_width(gload(V.row_len + row), seg);
     int kt = 0;
-    if (len > 0) {
-        const int xa = gload(V.row_xa + row), y = row % V.ny, z = row / V.ny, kf = gload(V.kfirst + p);
-        const int kend = gload(kglob + p);
+    if (tw > 0) {
+        const int tx0 = gload(V.row_xa + row) + kWave * seg, y = row % V.ny, z = row / V.ny;
+        const int kf = gload(V.kfirst + p), kend = gload(kglob + p);
         for (int k = kf; k < kend; ++k) {
             const int4 d = gload(V.pstep + (size_t)p * V.LP + k);
             const int ys = y - d.y, zs = z - d.z;
             if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
             const int srow = zs * V.ny + ys;
-            const int slen = gload(V.row_len + srow);
-            if (slen == 0 || k - kf >= gload(V.rinfo + (size_t)srow * R1 + p).cnt) continue;
-            const int sxa = gload(V.row_xa + srow);
-            if (sxa + d.x < xa + len && xa < sxa + slen + d.x) kt = k + 1;
+            const int slen = gload(V.row_len + srow), sxa = gload(V.row_xa + srow);
+            for (int ss = 0; ss * kWave < slen; ++ss) {
+                const int sx0 = sxa + kWave * ss, sw = segment_width(slen, ss);
+                if (sx0 + d.x < tx0 + tw && tx0 < sx0 + sw + d.x &&
+                    k - kf < gload(V.rinfo + ((size_t)srow * nseg + ss) * R1 + p).cnt) {
+                    kt = k + 1;
+                    break;
+                }
+            }
         }
     }
     V.rinfo[id].kt = kt;
 }
 
-// Target-major layout: row t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
-// source deposits into it (the near block: all 125 bins); planes over the row's active range.
-__global__ void k_target_totals(Vol V, long long* row_tot) {
-    const int rows = V.ny * V.nz, R1 = V.n_rays + 1;
-    const int row = blockIdx.x * blockDim.x + threadIdx.x;
-    if (row >= rows) return;
-    const int len = gload(V.row_len + row), xa = gload(V.row_xa + row);
+// Target-major layout: segment t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
+// source deposits into it (the near block: all 125 bins); planes over the segment's x range.
+__global__ void k_target_totals(Vol V, long long* unit_tot) {
+    const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
+    const int unit = blockIdx.x * blockDim.x + threadIdx.x;
+    if (unit >= units) return;
+    const int row = unit / V.nseg, seg = unit % V.nseg;
+    const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
     long long off = 0;
     for (int p = 0; p < R1; ++p) {
-        const int kt = V.rinfo[(size_t)row * R1 + p].kt;
-        const int cnt = len == 0 ? 0 : (p == 0 ? kNQ : max(0, kt - gload(V.kfirst + p)));
-        RowInfo* e = V.tinfo + (size_t)row * R1 + p;
+        const int kt = V.rinfo[(size_t)unit * R1 + p].kt;
+        const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : max(0, kt - gload(V.kfirst + p)));
+        RowInfo* e = V.tinfo + (size_t)unit * R1 + p;
         e->cnt = cnt;
         e->kt = kt;
         e->off = off;
-        e->xa = xa;
-        e->len = len;
-        off += (long long)cnt * len;
+        e->xa = x0;
+        e->len = w;
+        off += run_size(cnt, w);
     }
-    row_tot[row] = off;
+    unit_tot[unit] = off;
 }
 
-__global__ void k_target_offsets(Vol V, const long long* __restrict__ row_base) {
+__global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base) {
     const int R1 = V.n_rays + 1;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)V.ny * V.nz * R1) return;
-    V.tinfo[id].off += row_base[id / R1];
+    if (id >= (long long)V.ny * V.nz * V.nseg * R1) return;
+    V.tinfo[id].off += unit_base[id / R1];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -991,7 +1075,8 @@ __device__ inline float ray_planes(const WT* wrow, int len, int count, const int
 // out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
 // streamed; for step (p, k) lane t gathers g at t - D(p,k), a wave-uniform shift. The input
 // vector carries a margin wide enough for every stored shift, so no gather needs a bounds
-// check; zero weights (no source) are skipped by a select. Every voxel of the item is written.
+// check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
+// and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 template <class WT>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ gin,
@@ -999,32 +1084,28 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     if (item_id >= n_items) return;
     const int4 it = items[item_id];
-    if (it.w <= 0) return;
+    if (it.w == 0) return;
     const Vol& V = vols[it.x];
     const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
-    const RowInfo* tinfo = V.tinfo;
-    const int* plin = V.plin;
-    const int* kfirst = V.kfirst;
-    const WT* tb = (const WT*)V.tbuf;
     const int lane = threadIdx.x & (kWave - 1);
     const int row = it.y, x0 = it.z;
     float* o = out + V.vbase + (size_t)row * nx + x0;
-    const RowInfo own = gload(tinfo + (size_t)row * R1);
-    if (own.len == 0 || x0 + it.w <= own.xa || x0 >= own.xa + own.len) {
-        if (lane < it.w) o[lane] = 0.0f;
+    if (it.w < 0) {  // voxels outside the row's active range receive nothing (prc.cu:432 clears out)
+        if (lane < -it.w) o[lane] = 0.0f;
         return;
     }
+    const RowInfo* tinfo = V.tinfo + ((size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave) * R1;
+    const int* plin = V.plin;
+    const int* kfirst = V.kfirst;
+    const WT* tb = (const WT*)V.tbuf;
     const float* g0 = gin + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
-        const RowInfo ti = gload(tinfo + (size_t)row * R1 + p);
+        const RowInfo ti = gload(tinfo + p);
         const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
-        acc = ray_planes(tb + ti.off + (x0 - ti.xa), ti.len, ti.cnt, pl, g0, -1, lane, acc);
+        acc = ray_planes(tb + ti.off, ti.len, ti.cnt, pl, g0, -1, lane, acc);
     }
-    // Lanes outside the row's active range read neighbouring planes: they are off support and
-    // receive nothing (prc.cu:432 clears out; deposits land on support voxels only).
-    const int x = x0 + lane;
-    if (lane < it.w) o[lane] = (x >= own.xa && x < own.xa + own.len) ? acc : 0.0f;
+    if (lane < it.w) o[lane] = acc;  // lanes past the segment read the next plane: not written
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
@@ -1039,19 +1120,19 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     if (it.w <= 0) return;
     const Vol& V = vols[it.x];
     const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
-    const RowInfo* rinfo = V.rinfo;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, x0 = it.z;
+    const RowInfo* rinfo = V.rinfo + ((size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave) * R1;
     const int* plin = V.plin;
     const int* kfirst = V.kfirst;
     const WT* wb = (const WT*)V.wbuf;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int row = it.y, x0 = it.z;
     const bool valid = lane < it.w;
     const float* s0 = in + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
-        const RowInfo ri = gload(rinfo + (size_t)row * R1 + p);
+        const RowInfo ri = gload(rinfo + p);
         const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
-        acc = ray_planes(wb + ri.off + (x0 - ri.xa), ri.len, ri.cnt, pl, s0, 1, lane, acc);
+        acc = ray_planes(wb + ri.off, ri.len, ri.cnt, pl, s0, 1, lane, acc);
     }
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     if (valid && gload(V.support + loc)) {
@@ -1377,9 +1458,11 @@ struct prc_projector {
     long long* row_base = nullptr;
     long long* total = nullptr;
     int4* items_act = nullptr;  // active row segments (march, adjoint)
-    int4* items_all = nullptr;  // every row segment (forward, streaming)
-    int n_act = 0, n_all = 0;
-    std::vector<int4> h_items_act, h_items_all;
+    int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
+    int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
+    int n_act = 0, n_all = 0, n_fwd = 0;
+    std::vector<int4> h_items_act, h_items_all, h_items_fwd;
+    std::vector<int> h_row_xa, h_row_len;  // active x range of every row
     std::vector<unsigned char> h_support;
     unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
     // Near-block memoisation (uniform 5x5x5 neighbourhoods): per-voxel class (-2 inactive,
@@ -1421,6 +1504,10 @@ void build_items(prc_projector* A) {
     const int nx = A->g.nx, rows = A->g.ny * A->g.nz;
     A->h_items_act.clear();
     A->h_items_all.clear();
+    A->h_items_fwd.clear();
+    auto zeros = [&](int row, int b, int e) {  // [b, e) of a row, as zero pieces of the forward
+        for (int x0 = b; x0 < e; x0 += kWave) A->h_items_fwd.push_back(make_int4(0, row, x0, -std::min(kWave, e - x0)));
+    };
     std::vector<int> xa(rows, 0), len(rows, 0);
     for (int row = 0; row < rows; ++row) {
         const unsigned char* s = A->h_support.data() + (size_t)row * nx;
@@ -1432,15 +1519,24 @@ void build_items(prc_projector* A) {
             }
         if (lo >= 0) {
             xa[row] = lo, len[row] = hi - lo + 1;
-            for (int x0 = lo; x0 <= hi; x0 += kWave)
+            zeros(row, 0, lo);
+            for (int x0 = lo; x0 <= hi; x0 += kWave) {
                 A->h_items_act.push_back(make_int4(0, row, x0, std::min(kWave, hi + 1 - x0)));
+                A->h_items_fwd.push_back(A->h_items_act.back());
+            }
+            zeros(row, hi + 1, nx);
+        } else {
+            zeros(row, 0, nx);
         }
         for (int x0 = 0; x0 < nx; x0 += kWave) A->h_items_all.push_back(make_int4(0, row, x0, std::min(kWave, nx - x0)));
     }
     while (A->h_items_act.size() % kItemsPerBlock) A->h_items_act.push_back(make_int4(0, 0, 0, 0));
     while (A->h_items_all.size() % kItemsPerBlock) A->h_items_all.push_back(make_int4(0, 0, 0, 0));
+    while (A->h_items_fwd.size() % kItemsPerBlock) A->h_items_fwd.push_back(make_int4(0, 0, 0, 0));
     A->n_act = (int)A->h_items_act.size();
     A->n_all = (int)A->h_items_all.size();
+    A->n_fwd = (int)A->h_items_fwd.size();
+    A->h_row_xa = xa, A->h_row_len = len;
     A->row_xa = dalloc<int>(rows);
     A->row_len = dalloc<int>(rows);
     HIPCHK(hipMemcpy(A->row_xa, xa.data(), rows * sizeof(int), hipMemcpyHostToDevice));
@@ -1449,6 +1545,8 @@ void build_items(prc_projector* A) {
     A->items_all = dalloc<int4>(A->n_all);
     HIPCHK(hipMemcpy(A->items_act, A->h_items_act.data(), A->n_act * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(A->items_all, A->h_items_all.data(), A->n_all * sizeof(int4), hipMemcpyHostToDevice));
+    A->items_fwd = dalloc<int4>(A->n_fwd);
+    HIPCHK(hipMemcpy(A->items_fwd, A->h_items_fwd.data(), A->n_fwd * sizeof(int4), hipMemcpyHostToDevice));
     long long n_rows = 0;
     for (int row = 0; row < rows; ++row) n_rows += len[row] > 0;
     A->stats.n_rows = n_rows;
@@ -1554,8 +1652,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LP = std::max(L, kNQ);
     A->fstep = dalloc<FarStep>((size_t)R * L);
     A->pstep = dalloc<int4>((size_t)R1 * LP);
-    A->plin = dalloc<int>((size_t)R1 * LP + 16);  // slack for the group-wide scalar loads
-    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + 16) * sizeof(int)));
+    A->plin = dalloc<int>((size_t)R1 * LP + kProjGroup);  // slack for the group-wide scalar loads
+    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + kProjGroup) * sizeof(int)));
     A->kfirst = dalloc<int>(R1);
     A->kglob = dalloc<int>(R1);
     A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
@@ -1580,11 +1678,12 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
 
     build_items(A);
     V.row_xa = A->row_xa, V.row_len = A->row_len;
-    const size_t rows = (size_t)g.ny * g.nz;
-    A->rinfo = dalloc<RowInfo>(rows * R1);
-    A->tinfo = dalloc<RowInfo>(rows * R1);
-    A->row_tot = dalloc<long long>(rows);
-    A->row_base = dalloc<long long>(rows);
+    V.nseg = (g.nx + kWave - 1) / kWave;
+    const size_t units = (size_t)g.ny * g.nz * V.nseg;
+    A->rinfo = dalloc<RowInfo>(units * R1);
+    A->tinfo = dalloc<RowInfo>(units * R1);
+    A->row_tot = dalloc<long long>(units);
+    A->row_base = dalloc<long long>(units);
     A->total = dalloc<long long>(2);
     V.rinfo = A->rinfo, V.tinfo = A->tinfo;
     V.wbuf = nullptr, V.tbuf = nullptr;
@@ -1611,9 +1710,9 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
 // for the forward) and W. One host synchronisation (the buffer sizes); buffers are reused.
 void normalise_projector(prc_projector* A) {
     Vol& V = A->hv;
-    const size_t rows = (size_t)V.ny * V.nz;
+    const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1;
-    HIPCHK(hipMemsetAsync(A->rinfo, 0, rows * R1 * sizeof(RowInfo), 0));
+    HIPCHK(hipMemsetAsync(A->rinfo, 0, units * R1 * sizeof(RowInfo), 0));
     HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), 0));
     HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), 0));
     if (A->cls) {
@@ -1631,14 +1730,14 @@ void normalise_projector(prc_projector* A) {
     else launch_far<float>(A, 1);
     {
         ProfScope ps("layout", 0);
-        k_row_totals<<<blocks_for((long long)rows, 256), 256>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)rows, A->total);
-        k_row_offsets<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->row_base);
+        k_row_totals<<<blocks_for((long long)units, 256), 256>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)units, A->total);
+        k_row_offsets<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->row_base);
         k_kglob<<<R1, 256>>>(V, A->kglob);
-        k_forward_bounds<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->kglob);
-        k_target_totals<<<blocks_for((long long)rows, 256), 256>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)rows, A->total + 1);
-        k_target_offsets<<<blocks_for((long long)rows * R1, 256), 256>>>(V, A->row_base);
+        k_forward_bounds<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->kglob);
+        k_target_totals<<<blocks_for((long long)units, 256), 256>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)units, A->total + 1);
+        k_target_offsets<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->row_base);
         HIPCHK(hipGetLastError());
     }
     long long total[2] = {0, 0};
@@ -1676,30 +1775,52 @@ void normalise_projector(prc_projector* A) {
     }
 }
 
-// Host replay of every index range the forward and adjoint read (PRC_AUDIT=1): plane reads
-// must stay inside the buffer's one-wave slack, offset-table reads inside its 8 spare entries,
-// and every gather shift inside the vectors' margin. Used by the tests before any operator
-// launch; fails instead of launching a kernel that could read out of bounds.
+// Host replay of the layout and of every index range the forward and adjoint read (PRC_AUDIT=1):
+// each (segment, ray) run must follow the previous one in the buffer, plane reads must stay
+// inside the buffer's one-wave slack, offset-table reads inside its spare entries, and every
+// gather shift inside the vectors' margin. Used by the tests before any operator launch; fails
+// instead of launching a kernel that could read out of bounds.
 void audit_layout(prc_projector* A) {
     const Vol& V = A->hv;
-    const size_t rows = (size_t)V.ny * V.nz;
+    const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1, LP = V.LP;
-    std::vector<RowInfo> ri(rows * R1), ti(rows * R1);
-    std::vector<int> kf(R1), pl((size_t)R1 * LP + 16);
+    std::vector<RowInfo> ri(units * R1), ti(units * R1);
+    std::vector<int> kf(R1), pl((size_t)R1 * LP + kProjGroup);
     HIPCHK(hipMemcpy(ri.data(), A->rinfo, ri.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ti.data(), A->tinfo, ti.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(kf.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(pl.data(), A->plin, pl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    auto check_layout = [&](const std::vector<RowInfo>& info, long long total, const char* what) {
+        long long next = 0;
+        for (size_t u = 0; u < units; ++u) {
+            const int row = (int)(u / V.nseg), seg = (int)(u % V.nseg);
+            const int w = std::min(kWave, std::max(0, A->h_row_len[row] - kWave * seg));
+            for (int p = 0; p < R1; ++p) {
+                const RowInfo& e = info[u * R1 + p];
+                if (e.off != next || e.len != w || (w > 0 && e.xa != A->h_row_xa[row] + kWave * seg) || e.cnt < 0)
+                    fail(PRC_ERR_INVALID, "audit: %s layout of segment %zu ray %d is inconsistent", what, u, p);
+                next += run_size(e.cnt, e.len);
+            }
+        }
+        if (next != total) fail(PRC_ERR_INVALID, "audit: %s layout holds %lld entries, not %lld", what, next, total);
+    };
+    check_layout(ri, A->stats.weight_entries, "source-major");
+    check_layout(ti, A->stats.target_entries, "target-major");
     const long long shift_max = A->margin - kWave;
-    auto check_item = [&](const std::vector<RowInfo>& info, long long total, int row, int x0, const char* what) {
+    auto check_item = [&](const std::vector<RowInfo>& info, long long total, const int4& it, const char* what) {
+        const int row = it.y, x0 = it.z;
+        const int seg = (x0 - A->h_row_xa[row]) / kWave;
+        if (A->h_row_len[row] <= 0 || (x0 - A->h_row_xa[row]) % kWave || seg < 0 || seg >= V.nseg)
+            fail(PRC_ERR_INVALID, "audit: %s item at row %d x %d is not a segment", what, row, x0);
         for (int p = 0; p < R1; ++p) {
-            const RowInfo& e = info[(size_t)row * R1 + p];
+            const RowInfo& e = info[((size_t)row * V.nseg + seg) * R1 + p];
+            if (e.xa != x0 || e.len != it.w) fail(PRC_ERR_INVALID, "audit: %s item and segment disagree", what);
             if (e.cnt <= 0) continue;
-            const long long lo = e.off + (x0 - e.xa);
-            const long long hi = e.off + (long long)(e.cnt - 1) * e.len + (x0 - e.xa) + kWave - 1;
-            if (lo < -kWave || hi >= total + kWave)
-                fail(PRC_ERR_INVALID, "audit: %s planes of row %d ray %d read [%lld, %lld] outside [%d, %lld)", what,
-                     row, p, lo, hi, -kWave, total + kWave);
+            const long long lo = e.off;
+            const long long hi = e.off + (long long)(e.cnt - 1) * e.len + kWave - 1;
+            if (lo < 0 || hi >= total + kWave)
+                fail(PRC_ERR_INVALID, "audit: %s planes of row %d ray %d read [%lld, %lld] outside [0, %lld)", what,
+                     row, p, lo, hi, total + kWave);
             const long long last = (long long)p * LP + kf[p] + ((e.cnt - 1) / kProjGroup) * kProjGroup + kProjGroup - 1;
             if (kf[p] + e.cnt > LP || last >= (long long)pl.size())
                 fail(PRC_ERR_INVALID, "audit: %s offsets of ray %d read past the table", what, p);
@@ -1710,15 +1831,10 @@ void audit_layout(prc_projector* A) {
             }
         }
     };
-    for (const int4& it : A->h_items_all) {
-        if (it.w <= 0) continue;
-        const RowInfo& own = ti[(size_t)it.y * R1];
-        if (own.len == 0 || it.z + it.w <= own.xa || it.z >= own.xa + own.len) continue;
-        if (it.z - own.xa < -(kWave - 1)) fail(PRC_ERR_INVALID, "audit: forward segment starts too far left");
-        check_item(ti, A->stats.target_entries, it.y, it.z, "forward");
-    }
+    for (const int4& it : A->h_items_fwd)
+        if (it.w > 0) check_item(ti, A->stats.target_entries, it, "forward");
     for (const int4& it : A->h_items_act)
-        if (it.w > 0) check_item(ri, A->stats.weight_entries, it.y, it.z, "adjoint");
+        if (it.w > 0) check_item(ri, A->stats.weight_entries, it, "adjoint");
 }
 
 void read_counters(prc_projector* A) {
@@ -1732,7 +1848,7 @@ void read_counters(prc_projector* A) {
 void release(prc_projector* A) {
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
-    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->counters), dfree(A->scratch_in);
+    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
@@ -1779,7 +1895,8 @@ struct Batch {
     const Vol* dv = nullptr;
     const int4* items_all = nullptr;
     const int4* items_act = nullptr;
-    int n_all = 0, n_act = 0, n_vols = 1, wbytes = 4;
+    const int4* items_fwd = nullptr;
+    int n_all = 0, n_act = 0, n_fwd = 0, n_vols = 1, wbytes = 4;
     const int* item_begin = nullptr;  // per-volume ranges of items_all
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
@@ -1812,7 +1929,7 @@ void batch_iteration(const Batch& B) {
             k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
         }
     }
-    run_forward(B.dv, B.items_all, B.n_all, B.wbytes, B.gf, B.est, B.stream);
+    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.stream);
     {
         ProfScope ps("ratio", B.stream);
         k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
@@ -2032,7 +2149,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_all, A->n_all, A->wbytes, g, dev_out, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, 0);
     });
 }
 
@@ -2162,6 +2279,7 @@ static Batch solver_batch(prc_solver* S) {
     Batch B;
     prc_projector* A = S->A;
     B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
+    B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     B.sc = S->sc, B.partial = S->partial;
@@ -2236,7 +2354,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             b->A.assign(projectors, projectors + n);
             b->vols.resize(n);
             b->sc0.resize(n);
-            std::vector<int4> all, act;
+            std::vector<int4> all, act, fwd;
             std::vector<int> begin(n + 1, 0);
             const int wbytes = projectors[0] ? projectors[0]->wbytes : 4;
             const bool accel = params[0].accelerate != 0;
@@ -2253,6 +2371,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
                 begin[v] = (int)all.size();
                 for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
                 for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
+                for (const int4& it : A->h_items_fwd) fwd.push_back(make_int4(v, it.y, it.z, it.w));
                 b->total += A->n + 2 * (size_t)A->margin;
             }
             begin[n] = (int)all.size();
@@ -2269,11 +2388,14 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             b->dv = (Vol*)own(dalloc<Vol>(n));
             int4* d_all = (int4*)own(dalloc<int4>(all.size()));
             int4* d_act = (int4*)own(dalloc<int4>(std::max<size_t>(act.size(), 1)));
+            int4* d_fwd = (int4*)own(dalloc<int4>(std::max<size_t>(fwd.size(), 1)));
+            HIPCHK(hipMemcpy(d_fwd, fwd.data(), fwd.size() * sizeof(int4), hipMemcpyHostToDevice));
             int* d_begin = (int*)own(dalloc<int>(n + 1));
             HIPCHK(hipMemcpy(d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_act, act.data(), act.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
             B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
+            B.items_fwd = d_fwd, B.n_fwd = (int)fwd.size();
             B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
             B.g = (float*)own(dalloc<float>(b->total));
             B.gf = (float*)own(dalloc<float>(b->total));
