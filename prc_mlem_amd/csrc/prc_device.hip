@@ -30,6 +30,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <new>
 #include <string>
 #include <vector>
@@ -40,6 +41,7 @@ namespace {
 
 constexpr int kBins = PRC_RANGE_BINS;    // prc.cu:61
 constexpr int kNearRays = PRC_NEAR_RAYS; // prc.cu:104
+static_assert(kNearRays % 2 == 0, "the near march takes rays in pairs");
 constexpr int kNQ = 125;                 // 5x5x5 near block (prc.cu:103)
 constexpr int kNearMax = 8;              // a near ray visits at most 7 block voxels
 constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
@@ -86,7 +88,9 @@ Isotope renormalised(int idx) {
 // without contraction, as the reference's host table build does.
 __host__ __device__ inline float survival(const Branch* br, int n, float r) {
     float s = 0.0f;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // at most three branches; a fixed trip count keeps br in registers
+        if (i >= n) break;
         const float x = br[i].b * r, y = br[i].c * r;
         const float p = powf(br[i].a * r, 1.0f + br[i].f);
         const float q = x * x * sqrtf(x);
@@ -162,6 +166,8 @@ struct Vol {            // device descriptor of one volume
     Branch br[3];
     size_t vbase;       // offset of this volume in concatenated voxel vectors
     const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, +inf halo
+    const float* rho_base;  // start of the allocation holding rho_p and its +inf margins
+    long long rho_bytes;    // size of that allocation
     const float* table; // kBins + 1
     const unsigned char* support;
     float* W;           // per voxel
@@ -231,6 +237,28 @@ __device__ inline float lookup(const float* tab, float scale, float w) {  // prc
     const int i = min((int)u, kBins - 1);
     const float lo = tab[i], hi = tab[i + 1];
     return fmaf(fminf(u - (float)i, 1.0f), hi - lo, lo);
+}
+
+// Buffer-resource load of a float at a 32-bit byte offset from a wave-uniform base: the address
+// costs one 32-bit add instead of 64-bit arithmetic. Descriptor word 3 as for gfx9-family chips.
+__device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* base, long long bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)min(bytes, 0x7fffffffLL),
+                                             0x00020000);
+}
+__device__ inline float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+
+// lookup() for the march, on the LDS table with its extra entry. The march only asks for
+// 0 <= w <= cutoff, so u = w * scale <= kBins to within one rounding: for u < kBins both forms take
+// the same bin and fraction; for u >= kBins lookup() returns lo + 1 * (T[kBins] - lo) with lo =
+// T[kBins - 1], which is T[kBins] exactly (the two neighbouring survivals are within a factor of
+// two, so the difference is exact), and this form returns T[kBins] + f * 0.
+__device__ inline float march_lookup(const float* tab, float scale, float w) {
+    const float u = w * scale;
+    const int i = (int)u;
+    const float lo = tab[i], hi = tab[i + 1];
+    return fmaf(u - (float)i, hi - lo, lo);
 }
 
 // Index of the calling wave inside its block, known to the compiler as wave-uniform so that
@@ -357,9 +385,10 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirs
 // Normalisation march (prc.cu:178-237 with NormOp): one wave = 64 consecutive sources of a row.
 // Per step the DDA offset is wave-uniform, so density reads are (uniform base + lane).
 
+// March state of one lane. A dead lane holds w = cutoff and surv = S(cutoff): every further step
+// then leaves it unchanged and deposits nothing, so no separate alive flag is carried.
 struct Lane {
-    float w = 0.0f, surv = 1.0f, air = 0.0f;
-    bool alive;
+    float w, surv, air;
 };
 
 // March parameters of a volume, copied out of the descriptor once per wave: the kernels store
@@ -369,34 +398,43 @@ struct MarchConst {
     float cutoff, scale, support_rho, air_cap;
     int nb;
     Branch br[3];
-    __device__ explicit MarchConst(const Vol& V)
+    float s_cut;  // S(cutoff) as the step evaluates it: the survival a dead lane keeps
+    template <bool EXACT>
+    __device__ MarchConst(const Vol& V, const float* tab, std::integral_constant<bool, EXACT>)
         : cutoff(V.cutoff), scale(V.scale), support_rho(V.support_rho), air_cap(V.air_cap), nb(V.nb),
-          br{V.br[0], V.br[1], V.br[2]} {}
+          br{V.br[0], V.br[1], V.br[2]} {
+        s_cut = EXACT ? survival(br, nb, cutoff) : march_lookup(tab, scale, cutoff);
+    }
+    __device__ Lane start(bool act) const { return Lane{act ? 0.0f : cutoff, act ? 1.0f : s_cut, 0.0f}; }
+    __device__ bool alive(const Lane& st) const { return st.w < cutoff; }
 };
 
 // One DDA step of prc.cu:146-163 for the current lane; returns the deposit (0 if none). Written
-// with selects instead of branches (a dead lane computes and discards): the ray leaves the grid
-// at the +inf halo, dies in air after air_cap mm, and dies when the range reaches the cutoff.
+// with selects instead of branches: the ray leaves the grid at the +inf halo, dies in air after
+// air_cap mm, and dies when the range reaches the cutoff. A dead lane (w = cutoff) computes
+// wn = cutoff and sn = S(cutoff) = surv, so it deposits nothing and stays dead.
 template <bool EXACT>
 __device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_here, const MarchConst& V,
                                    const float* tab, unsigned& visits) {
-    const bool inside = st.alive && rho < INFINITY;
-    visits += inside ? 1u : 0u;
+    const bool inside = rho < INFINITY;
+    visits += (inside && st.w < V.cutoff) ? 1u : 0u;
     const float wn = fminf(fmaf(rho, seg, st.w), V.cutoff);
-    const float sn = EXACT ? survival(V.br, V.nb, wn) : lookup(tab, V.scale, wn);
+    const float sn = EXACT ? survival(V.br, V.nb, wn) : march_lookup(tab, V.scale, wn);
     const bool tissue = rho >= V.support_rho;
     const float air = tissue ? 0.0f : st.air + seg;
-    const bool live = inside && (tissue || !(air > V.air_cap));
-    const float dep = (live && tissue && deposit_here && st.surv > sn) ? st.surv - sn : 0.0f;
-    st.w = live ? wn : st.w;
-    st.surv = live ? sn : st.surv;
-    st.air = live ? air : st.air;
-    st.alive = live && !(wn >= V.cutoff);
+    const float dep = (inside && tissue && deposit_here && st.surv > sn) ? st.surv - sn : 0.0f;
+    const bool dies = !inside || (!tissue && air > V.air_cap);
+    st.w = dies ? V.cutoff : wn;
+    st.surv = dies ? V.s_cut : sn;
+    st.air = air;
     return dep;
 }
 
+// The march's copy of the table in LDS carries one extra entry, tab[kBins + 1] = tab[kBins], so
+// that march_lookup needs no clamps.
+constexpr int kTabLds = kBins + 2;
 __device__ inline void load_table(float* tab, const Vol& V) {
-    for (int j = threadIdx.x; j <= kBins; j += blockDim.x) tab[j] = gload(V.table + j);
+    for (int j = threadIdx.x; j < kTabLds; j += blockDim.x) tab[j] = gload(V.table + min(j, kBins));
     __syncthreads();
 }
 
@@ -452,34 +490,51 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
         if (i < 24) a5[i] = 0.0f;
         if (i < 8) a6[i] = 0.0f;
     }
-    const MarchConst M(V);
-    for (int r = 0; r < kNearRays; ++r) {
-        const int len = gload(V.nlen + r);
-        const NearStep* nsr = V.nstep + r * kNearMax;
+    const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
+    // Two rays at a time: their steps are independent, so the dependent chain of one (depth,
+    // table lookup, survival) overlaps the other's. Each bin still receives the rays' deposits in
+    // ray order (a ray visits a voxel once), so the sums are those of one ray after another.
+    // Steps past a ray's length see +inf density and a zero step entry: they deposit exactly 0
+    // into bin 0 of their shell and count no visit.
+    for (int r = 0; r < kNearRays; r += 2) {
+        int len[2];
+        NearStep ns[2][7];
+        float rv[2][7];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            len[h] = gload(V.nlen + r + h);
+            const NearStep* nsr = V.nstep + (r + h) * kNearMax;  // zero past the ray's length
+#pragma unroll
+            for (int s = 0; s < 7; ++s) ns[h][s] = gload(nsr + s);
+        }
         // The whole ray stays inside the 5x5x5 block: load its densities up front (the +inf
         // margin around the padded density keeps every address in bounds).
-        NearStep ns[7];
-        float rv[7];
 #pragma unroll
-        for (int s = 0; s < 7; ++s) ns[s] = s < len ? gload(nsr + s) : NearStep{0.0f, 0, 0, 0};
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int s = 0; s < 7; ++s) rv[s] = (s < len && act) ? rho_at(ns[s].dp) : INFINITY;
-        Lane st;
-        st.alive = act;
-        auto step = [&](int s) { return march_step<EXACT>(st, rv[s], ns[s].seg, true, M, tab, visits); };
-        a0 += step(0);  // every near ray starts inside the source voxel
-        if (len > 1) { const float d = step(1); a1[ns[1].loc] += d; }
-        if (len > 2) { const float d = step(2); a2[ns[2].loc] += d; }
-        if (len > 3) { const float d = step(3); a3[ns[3].loc] += d; }
-        if (len > 4) {  // both halves are updated, branch-free, to keep the registers in place
-            const float d = step(4);
-            const int loc = ns[4].loc;
+            for (int s = 0; s < 7; ++s) rv[h][s] = (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
+        Lane st[2] = {M.start(act), M.start(act)};
+        auto step = [&](int h, int s) { return march_step<EXACT>(st[h], rv[h][s], ns[h][s].seg, true, M, tab, visits); };
+#pragma unroll
+        for (int h = 0; h < 2; ++h) a0 += step(h, 0);  // every near ray starts inside the source voxel
+#pragma unroll
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 1); a1[ns[h][1].loc] += d; }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 2); a2[ns[h][2].loc] += d; }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 3); a3[ns[h][3].loc] += d; }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {  // both halves are updated, branch-free, to keep the registers in place
+            const float d = step(h, 4);
+            const int loc = ns[h][4].loc;
             const bool lo = loc < 18;
             a4l[lo ? loc : 0] += lo ? d : 0.0f;
             a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
         }
-        if (len > 5) { const float d = step(5); a5[ns[5].loc] += d; }
-        if (len > 6) { const float d = step(6); a6[ns[6].loc] += d; }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 5); a5[ns[h][5].loc] += d; }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 6); a6[ns[h][6].loc] += d; }
     }
 }
 
@@ -498,7 +553,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
 template <bool EXACT, class WT>
 __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                int n_items, unsigned long long* vrs) {
-    __shared__ float tab[kBins + 1];
+    __shared__ float tab[kTabLds];
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     const int4 it = items[min(item_id, n_items - 1)];
     const Vol& V = vols[it.x];
@@ -553,7 +608,7 @@ template <bool EXACT, class WT>
 __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols, const int2* __restrict__ list,
                                                     int n_list, float* __restrict__ tmpl,
                                                     float* __restrict__ tmpl_sum, unsigned long long* vrs) {
-    __shared__ float tab[kBins + 1];
+    __shared__ float tab[kTabLds];
     const Vol& V = vols[0];
     if (!EXACT) load_table(tab, V);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -754,13 +809,13 @@ __device__ inline long long lane_of(long long v, int l) {
 template <int PASS, bool EXACT, class WT>
 __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                               int n_items, unsigned long long* vrs, unsigned long long* ndep) {
-    __shared__ float tab[kBins + 1];
+    __shared__ float tab[kTabLds];
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     const int4 it = items[min(item_id, n_items - 1)];
     const Vol& V = vols[it.x];
     if (!EXACT) load_table(tab, V);
     if (item_id >= n_items || it.w <= 0) return;
-    const MarchConst M(V);
+    const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
     const int ny = V.ny, nz = V.nz, L = V.L, n_rays = V.n_rays, R1 = n_rays + 1;
     const FarStep* fstep = V.fstep;
     const int* kfirst = V.kfirst;
@@ -773,7 +828,11 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     const bool valid = lane < it.w;
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
-    const float* rho0 = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane;
+    // Density reads go through a buffer descriptor over the whole padded allocation: lane index
+    // plus the step's offset, as one 32-bit add (every index stays inside the allocation).
+    const __amdgpu_buffer_rsrc_t rho_rsrc = buffer_of(V.rho_base, V.rho_bytes);
+    const int rho_idx = (int)(V.rho_p - V.rho_base) +
+                        (int)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
     const int nseg = V.nseg;
     const size_t unit = (size_t)row * nseg + (x0 - gload(V.row_xa + row)) / kWave;  // this segment
     const int* row_xa = V.row_xa;
@@ -795,8 +854,7 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
         const int len = PASS == 2 ? it.w : 0;  // plane width = the segment's width
         // PASS 2 marches only up to the row's last stored step.
         const int send = PASS == 2 ? kf + cnt : L;
-        Lane st;
-        st.alive = act;
+        Lane st = M.start(act);
         int last = -1;
         int s = 0;
         bool going = __any(act);
@@ -818,13 +876,14 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
                 // Density loads of the group first. A lane alive at the group start is inside the
                 // grid, so at most kFarGroup voxels outside it by the group's end: inside the +inf
                 // margin of the padded density.
-                // Every lane loads (a dead lane its own voxel), so the loads issue back to back.
+                // Every lane loads (a dead lane its own voxel, whose density it ignores), so the
+                // loads issue back to back.
+                const int go = M.alive(st) ? -1 : 0;  // dead lanes read their own voxel
                 float seg[kFarGroup], rv[kFarGroup];
 #pragma unroll
                 for (int u = 0; u < kFarGroup; ++u) {
                     seg[u] = lane_of(fl.seg, j + u);
-                    const float v = gload(rho0 + (st.alive ? lane_of(fl.dp, j + u) : 0));
-                    rv[u] = st.alive ? v : INFINITY;
+                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + (lane_of(fl.dp, j + u) & go)) * 4);
                 }
                 // Steps past nst are marched too, without stores: past the row's last stored step
                 // no lane of the row deposits (that is what the count pass measured), and past the
@@ -860,7 +919,7 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
                     }
                 }
                 s = s0 + j + kFarGroup;
-                if (!__any(st.alive)) {
+                if (!__any(M.alive(st))) {
                     going = false;
                     break;
                 }
@@ -1077,17 +1136,37 @@ __device__ inline float ray_planes(const WT* wrow, int len, int count, const int
 // vector carries a margin wide enough for every stored shift, so no gather needs a bounds
 // check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
-template <class WT>
-__global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                                  int n_items, const float* __restrict__ gin,
-                                                  float* __restrict__ out) {
-    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
-    if (item_id >= n_items) return;
-    const int4 it = items[item_id];
-    if (it.w == 0) return;
-    const Vol& V = vols[it.x];
-    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
+// Work distribution of the projector passes. The hardware places block b on XCD b % 8, and each
+// XCD has its own L2. The item list (rows in z-major order) is cut into one contiguous chunk per
+// XCD; a wave takes items from its own XCD's chunk, so the gathered neighbourhoods of the rows an
+// XCD works on stay in that XCD's L2, and moves on to the next chunks once its own is drained
+// (load balance). The counters are cleared by k_reset_queue before every launch.
+constexpr int kXcds = 8;
+constexpr int kProjBlocks = 2048;  // resident blocks at full occupancy: 256 CUs x 8
+
+__global__ void k_reset_queue(int* queue) {
+    if (threadIdx.x < kXcds) queue[threadIdx.x] = 0;
+}
+
+__device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
     const int lane = threadIdx.x & (kWave - 1);
+    while (slot < kXcds) {
+        const int c = (xcd + slot) % kXcds;
+        const int b = (int)((long long)n_items * c / kXcds), e = (int)((long long)n_items * (c + 1) / kXcds);
+        int i = 0;
+        if (lane == 0) i = atomicAdd(queue + c, 1);
+        i = __builtin_amdgcn_readfirstlane(__shfl(i, 0));
+        if (b + i < e) return b + i;
+        ++slot;
+    }
+    return -1;
+}
+
+// out = A g where g already holds f / W (one item; see k_forward).
+template <class WT>
+__device__ inline void forward_item(const Vol& V, const int4 it, const float* __restrict__ gin,
+                                    float* __restrict__ out, int lane) {
+    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
     const int row = it.y, x0 = it.z;
     float* o = out + V.vbase + (size_t)row * nx + x0;
     if (it.w < 0) {  // voxels outside the row's active range receive nothing (prc.cu:432 clears out)
@@ -1108,19 +1187,28 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
     if (lane < it.w) o[lane] = acc;  // lanes past the segment read the next plane: not written
 }
 
-// out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
-// planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
+// out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
+// streamed; for step (p, k) lane t gathers g at t - D(p,k), a wave-uniform shift. The input
+// vector carries a margin wide enough for every stored shift, so no gather needs a bounds
+// check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
+// and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 template <class WT>
-__global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                                  int n_items, const float* __restrict__ in,
-                                                  float* __restrict__ out) {
-    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
-    if (item_id >= n_items) return;
-    const int4 it = items[item_id];
-    if (it.w <= 0) return;
-    const Vol& V = vols[it.x];
+__global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                  int n_items, const float* __restrict__ gin,
+                                                  float* __restrict__ out, int* __restrict__ queue) {
+    const int xcd = blockIdx.x % kXcds, lane = threadIdx.x & (kWave - 1);
+    int slot = 0;
+    for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
+        const int4 it = items[i];
+        if (it.w != 0) forward_item<WT>(vols[it.x], it, gin, out, lane);
+    }
+}
+
+// out_s = (sum_t w_st in_t) / W_s on active s (one item; see k_adjoint).
+template <class WT>
+__device__ inline void adjoint_item(const Vol& V, const int4 it, const float* __restrict__ in,
+                                    float* __restrict__ out, int lane) {
     const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
-    const int lane = threadIdx.x & (kWave - 1);
     const int row = it.y, x0 = it.z;
     const RowInfo* rinfo = V.rinfo + ((size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave) * R1;
     const int* plin = V.plin;
@@ -1138,6 +1226,20 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     if (valid && gload(V.support + loc)) {
         const float wv = gload(V.W + loc);
         out[V.vbase + loc] = wv > 0.0f ? acc / wv : 0.0f;
+    }
+}
+
+// out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
+// planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
+template <class WT>
+__global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                  int n_items, const float* __restrict__ in,
+                                                  float* __restrict__ out, int* __restrict__ queue) {
+    const int xcd = blockIdx.x % kXcds, lane = threadIdx.x & (kWave - 1);
+    int slot = 0;
+    for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
+        const int4 it = items[i];
+        if (it.w > 0) adjoint_item<WT>(vols[it.x], it, in, out, lane);
     }
 }
 
@@ -1460,6 +1562,7 @@ struct prc_projector {
     int4* items_act = nullptr;  // active row segments (march, adjoint)
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
+    int* queue = nullptr;       // work-queue counters of the projector passes
     int n_act = 0, n_all = 0, n_fwd = 0;
     std::vector<int4> h_items_act, h_items_all, h_items_fwd;
     std::vector<int> h_row_xa, h_row_len;  // active x range of every row
@@ -1546,6 +1649,7 @@ void build_items(prc_projector* A) {
     HIPCHK(hipMemcpy(A->items_act, A->h_items_act.data(), A->n_act * sizeof(int4), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(A->items_all, A->h_items_all.data(), A->n_all * sizeof(int4), hipMemcpyHostToDevice));
     A->items_fwd = dalloc<int4>(A->n_fwd);
+    A->queue = dalloc<int>(kXcds);
     HIPCHK(hipMemcpy(A->items_fwd, A->h_items_fwd.data(), A->n_fwd * sizeof(int4), hipMemcpyHostToDevice));
     long long n_rows = 0;
     for (int row = 0; row < rows; ++row) n_rows += len[row] > 0;
@@ -1602,6 +1706,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const size_t np = (size_t)nxp * nyp * nzp;
     // +inf margin of (kFarGroup + 1) padded slices on both sides for the march's prefetches.
     const size_t margin = (size_t)(kFarGroup + 1) * nxp * nyp;
+    if ((np + 2 * margin) * sizeof(float) > (size_t)INT_MAX)
+        fail(PRC_ERR_INVALID, "Volume too large for 32-bit density addressing");
     std::vector<float> rp(np + 2 * margin, INFINITY);
     A->h_support.assign(n, 0);
     long long n_active = 0;
@@ -1669,6 +1775,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     V.exact = (g.flags & PRC_FLAG_EXACT_RANGE) != 0;
     V.L = L, V.LP = LP, V.vbase = 0;
     V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
+    V.rho_base = A->rho_alloc, V.rho_bytes = (long long)(np + 2 * margin) * (long long)sizeof(float);
     V.fstep = A->fstep, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
     V.nlen = A->nlen;
 
@@ -1848,7 +1955,7 @@ void read_counters(prc_projector* A) {
 void release(prc_projector* A) {
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
-    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->counters), dfree(A->scratch_in);
+    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
@@ -1857,21 +1964,24 @@ void release(prc_projector* A) {
 }
 
 // Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
-void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out,
+// `queue` holds kXcds counters in device memory owned by the caller.
+void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out, int* queue,
                  hipStream_t s) {
-    const unsigned nb = blocks_for(n_items, kItemsPerBlock);
+    const unsigned nb = std::min<unsigned>(kProjBlocks, blocks_for(n_items, kItemsPerBlock));
+    k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("forward", s);
-    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
-    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out);
+    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
     HIPCHK(hipGetLastError());
 }
 
-void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out,
+void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out, int* queue,
                  hipStream_t s) {
-    const unsigned nb = blocks_for(n_items, kItemsPerBlock);
+    const unsigned nb = std::min<unsigned>(kProjBlocks, blocks_for(n_items, kItemsPerBlock));
+    k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("adjoint", s);
-    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
-    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out);
+    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
     HIPCHK(hipGetLastError());
 }
 
@@ -1897,6 +2007,7 @@ struct Batch {
     const int4* items_act = nullptr;
     const int4* items_fwd = nullptr;
     int n_all = 0, n_act = 0, n_fwd = 0, n_vols = 1, wbytes = 4;
+    int* queue = nullptr;  // work-queue counters of the projector passes
     const int* item_begin = nullptr;  // per-volume ranges of items_all
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
@@ -1929,12 +2040,12 @@ void batch_iteration(const Batch& B) {
             k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
         }
     }
-    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.stream);
+    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, B.stream);
     {
         ProfScope ps("ratio", B.stream);
         k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
     }
-    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.stream);
+    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, B.stream);
     {
         ProfScope ps("update", B.stream);
         k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
@@ -2149,7 +2260,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, A->queue, 0);
     });
 }
 
@@ -2162,7 +2273,7 @@ int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out)
         }
         float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
         HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, 0);
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, A->queue, 0);
     });
 }
 
@@ -2279,7 +2390,7 @@ static Batch solver_batch(prc_solver* S) {
     Batch B;
     prc_projector* A = S->A;
     B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
-    B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd;
+    B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd, B.queue = A->queue;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     B.sc = S->sc, B.partial = S->partial;
@@ -2396,6 +2507,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             HIPCHK(hipMemcpy(d_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
             B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
             B.items_fwd = d_fwd, B.n_fwd = (int)fwd.size();
+            B.queue = (int*)own(dalloc<int>(kXcds));
             B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
             B.g = (float*)own(dalloc<float>(b->total));
             B.gf = (float*)own(dalloc<float>(b->total));
