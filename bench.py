@@ -264,6 +264,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "projector_effective": projector,
     }
     out.update(result)
+    out["projector_tuning"] = prc.tuning_report()
     if fp32 is not None:
         out["fp32"] = fp32
     if mixed is not None:

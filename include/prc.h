@@ -174,6 +174,12 @@ int prc_batch_correct(int n, prc_projector* const* projectors, const float* cons
 int prc_profile_enable(int on);
 int prc_profile_report(char* json, size_t capacity);
 
+/* JSON array of the projector-pass tuning decisions made so far: per solver or batch, the mean
+ * launch time of each schedule of the forward and the adjoint ("grid": one wave per row segment in
+ * launch order; "queue": XCD-local work queues) and the one kept. No counterpart in the reference
+ * (prc.cu has one fixed launch shape); PRC_TUNE=0 disables tuning. */
+int prc_tuning_report(char* json, size_t capacity);
+
 #ifdef __cplusplus
 }
 #endif

@@ -94,6 +94,7 @@ SIGNATURES = {
                                     C.POINTER(SolveParams), C.c_int, C.POINTER(C.c_void_p)]),
     "prc_profile_enable": (C.c_int, [C.c_int]),
     "prc_profile_report": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "prc_tuning_report": (C.c_int, [C.c_char_p, C.c_size_t]),
 }
 
 _lib = None
@@ -336,6 +337,15 @@ def profile_report():
     import json
     buf = C.create_string_buffer(1 << 16)
     _check(lib().prc_profile_report(buf, len(buf)))
+    return json.loads(buf.value.decode())
+
+
+def tuning_report():
+    """Projector-pass tuning decisions: per solver or batch, the mean launch time of each schedule
+    of the forward and the adjoint and the one kept."""
+    import json
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().prc_tuning_report(buf, len(buf)))
     return json.loads(buf.value.decode())
 
 

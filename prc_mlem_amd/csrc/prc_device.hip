@@ -232,13 +232,6 @@ template <> struct WeightIO<__half> {
     __device__ static float to_float(__half v) { return __half2float(v); }
 };
 
-__device__ inline float lookup(const float* tab, float scale, float w) {  // prc.cu:74-80
-    const float u = w * scale;
-    const int i = min((int)u, kBins - 1);
-    const float lo = tab[i], hi = tab[i + 1];
-    return fmaf(fminf(u - (float)i, 1.0f), hi - lo, lo);
-}
-
 // Buffer-resource load of a float at a 32-bit byte offset from a wave-uniform base: the address
 // costs one 32-bit add instead of 64-bit arithmetic. Descriptor word 3 as for gfx9-family chips.
 __device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* base, long long bytes) {
@@ -249,11 +242,12 @@ __device__ inline float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) 
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
 
-// lookup() for the march, on the LDS table with its extra entry. The march only asks for
-// 0 <= w <= cutoff, so u = w * scale <= kBins to within one rounding: for u < kBins both forms take
-// the same bin and fraction; for u >= kBins lookup() returns lo + 1 * (T[kBins] - lo) with lo =
-// T[kBins - 1], which is T[kBins] exactly (the two neighbouring survivals are within a factor of
-// two, so the difference is exact), and this form returns T[kBins] + f * 0.
+// Table lookup of prc.cu:74-80 (u = w * scale; i = min(int(u), kBins - 1); lo + min(u - i, 1) *
+// (hi - lo)) on the LDS table with its extra entry tab[kBins + 1] = tab[kBins], without the clamps.
+// The march only asks for 0 <= w <= cutoff, so u <= kBins to within one rounding: for u < kBins
+// both forms take the same bin and fraction; for u >= kBins the reference form returns
+// lo + 1 * (T[kBins] - lo) with lo = T[kBins - 1], which is T[kBins] exactly (neighbouring
+// survivals are within a factor of two, so the difference is exact), and this one T[kBins] + f * 0.
 __device__ inline float march_lookup(const float* tab, float scale, float w) {
     const float u = w * scale;
     const int i = (int)u;
@@ -1192,11 +1186,20 @@ __device__ inline void forward_item(const Vol& V, const int4 it, const float* __
 // vector carries a margin wide enough for every stored shift, so no gather needs a bounds
 // check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
-template <class WT>
+// QUEUE selects the schedule: XCD-local work queues, or one wave per item in launch order.
+template <class WT, bool QUEUE>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ gin,
                                                   float* __restrict__ out, int* __restrict__ queue) {
-    const int xcd = blockIdx.x % kXcds, lane = threadIdx.x & (kWave - 1);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (!QUEUE) {
+        const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
+        if (i >= n_items) return;
+        const int4 it = items[i];
+        if (it.w != 0) forward_item<WT>(vols[it.x], it, gin, out, lane);
+        return;
+    }
+    const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
@@ -1231,11 +1234,19 @@ __device__ inline void adjoint_item(const Vol& V, const int4 it, const float* __
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
 // planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
-template <class WT>
+template <class WT, bool QUEUE>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const float* __restrict__ in,
                                                   float* __restrict__ out, int* __restrict__ queue) {
-    const int xcd = blockIdx.x % kXcds, lane = threadIdx.x & (kWave - 1);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (!QUEUE) {
+        const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
+        if (i >= n_items) return;
+        const int4 it = items[i];
+        if (it.w > 0) adjoint_item<WT>(vols[it.x], it, in, out, lane);
+        return;
+    }
+    const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
@@ -1563,6 +1574,8 @@ struct prc_projector {
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
     int* queue = nullptr;       // work-queue counters of the projector passes
+    int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
+    bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
     std::vector<int4> h_items_act, h_items_all, h_items_fwd;
     std::vector<int> h_row_xa, h_row_len;  // active x range of every row
@@ -1964,24 +1977,39 @@ void release(prc_projector* A) {
 }
 
 // Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
+// Schedules of the projector passes (picked per solver or batch by tune_passes).
+enum Sched { kSchedGrid = 0, kSchedQueue = 1 };
+
 // `queue` holds kXcds counters in device memory owned by the caller.
 void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out, int* queue,
-                 hipStream_t s) {
-    const unsigned nb = std::min<unsigned>(kProjBlocks, blocks_for(n_items, kItemsPerBlock));
-    k_reset_queue<<<1, 64, 0, s>>>(queue);
+                 int sched, hipStream_t s) {
+    const unsigned all = blocks_for(n_items, kItemsPerBlock);
+    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
+    if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("forward", s);
-    if (wbytes == 2) k_forward<__half><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
-    else k_forward<float><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+    if (wbytes == 2) {
+        if (sched == kSchedQueue) k_forward<__half, true><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+        else k_forward<__half, false><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+    } else {
+        if (sched == kSchedQueue) k_forward<float, true><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+        else k_forward<float, false><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
+    }
     HIPCHK(hipGetLastError());
 }
 
 void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out, int* queue,
-                 hipStream_t s) {
-    const unsigned nb = std::min<unsigned>(kProjBlocks, blocks_for(n_items, kItemsPerBlock));
-    k_reset_queue<<<1, 64, 0, s>>>(queue);
+                 int sched, hipStream_t s) {
+    const unsigned all = blocks_for(n_items, kItemsPerBlock);
+    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
+    if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("adjoint", s);
-    if (wbytes == 2) k_adjoint<__half><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    else k_adjoint<float><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    if (wbytes == 2) {
+        if (sched == kSchedQueue) k_adjoint<__half, true><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+        else k_adjoint<__half, false><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    } else {
+        if (sched == kSchedQueue) k_adjoint<float, true><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+        else k_adjoint<float, false><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -2008,6 +2036,7 @@ struct Batch {
     const int4* items_fwd = nullptr;
     int n_all = 0, n_act = 0, n_fwd = 0, n_vols = 1, wbytes = 4;
     int* queue = nullptr;  // work-queue counters of the projector passes
+    int fwd_sched = kSchedQueue, adj_sched = kSchedQueue;  // chosen by tune_passes
     const int* item_begin = nullptr;  // per-volume ranges of items_all
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
@@ -2040,12 +2069,12 @@ void batch_iteration(const Batch& B) {
             k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
         }
     }
-    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, B.stream);
+    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, B.fwd_sched, B.stream);
     {
         ProfScope ps("ratio", B.stream);
         k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
     }
-    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, B.stream);
+    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, B.adj_sched, B.stream);
     {
         ProfScope ps("update", B.stream);
         k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
@@ -2058,6 +2087,56 @@ void batch_iteration(const Batch& B) {
         if (B.accel) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 2);
     }
     HIPCHK(hipGetLastError());
+}
+
+// Autotuner of the projector passes: every schedule of the forward and the adjoint is timed on
+// the solver's or batch's own weights and buffers (the forward into the estimate buffer, the
+// adjoint into the work buffer; the caller resets both afterwards), and the faster is kept. The
+// winner depends on the batch: several volumes per launch favour the XCD-local queues, a single
+// volume the launch-order grid. PRC_TUNE=0 keeps the defaults.
+struct TuneRecord {
+    int vols, wbytes, items_fwd, items_adj;
+    float ms[2][2];  // [forward, adjoint][grid, queue], mean of two launches
+    int choice[2];
+};
+std::vector<TuneRecord> g_tuning;
+
+void tune_passes(Batch& B) {
+    const char* env = getenv("PRC_TUNE");
+    if (env && env[0] == '0') return;
+    const bool prof = g_prof_on;
+    g_prof_on = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {{0, 0}, {0, 0}}, {0, 0}};
+    try {
+        for (int op = 0; op < 2; ++op)
+            for (int sched = 0; sched < 2; ++sched) {
+                auto launch = [&] {
+                    if (op == 0) run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, sched, B.stream);
+                    else run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, sched, B.stream);
+                };
+                launch();  // warm-up
+                HIPCHK(hipEventRecord(e0, B.stream));
+                launch();
+                launch();
+                HIPCHK(hipEventRecord(e1, B.stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                r.ms[op][sched] = ms / 2;
+            }
+    } catch (...) {
+        (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+        g_prof_on = prof;
+        throw;
+    }
+    (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
+    g_prof_on = prof;
+    for (int op = 0; op < 2; ++op) r.choice[op] = r.ms[op][kSchedQueue] < r.ms[op][kSchedGrid] ? kSchedQueue : kSchedGrid;
+    B.fwd_sched = r.choice[0], B.adj_sched = r.choice[1];
+    g_tuning.push_back(r);
 }
 
 void batch_rotate(Batch& B) {  // prc.cu:697-700
@@ -2260,7 +2339,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, A->queue, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, A->queue, A->fwd_sched, 0);
     });
 }
 
@@ -2273,7 +2352,7 @@ int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out)
         }
         float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
         HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, A->queue, 0);
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, A->queue, A->adj_sched, 0);
     });
 }
 
@@ -2330,6 +2409,8 @@ int prc_projector_stats(prc_projector* A, prc_stats* stats) {
     });
 }
 
+static Batch solver_batch(prc_solver* S);
+
 int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_params* params, prc_solver** out) {
     return guarded([&] {
         if (!A || !host_g || !params || !out) fail(PRC_ERR_INVALID, "null argument");
@@ -2357,8 +2438,15 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
             int ib[2] = {0, A->n_all};
             S->item_begin = dalloc<int>(2);
             HIPCHK(hipMemcpy(S->item_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
-            const int rc = prc_solver_reset(S);
+            int rc = prc_solver_reset(S);
             if (rc) throw Failure{rc};
+            if (!A->tuned) {  // the projector's first solver tunes its passes
+                Batch B = solver_batch(S);
+                tune_passes(B);
+                A->fwd_sched = B.fwd_sched, A->adj_sched = B.adj_sched, A->tuned = true;
+                rc = prc_solver_reset(S);
+                if (rc) throw Failure{rc};
+            }
         } catch (...) {
             prc_solver_destroy(S);
             throw;
@@ -2391,6 +2479,7 @@ static Batch solver_batch(prc_solver* S) {
     prc_projector* A = S->A;
     B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
     B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd, B.queue = A->queue;
+    B.fwd_sched = A->fwd_sched, B.adj_sched = A->adj_sched;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     B.sc = S->sc, B.partial = S->partial;
@@ -2517,6 +2606,9 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             B.sc = (Scal*)own(dalloc<Scal>(n));
             B.partial = (double*)own(dalloc<double>(2 * all.size()));
             HIPCHK(hipStreamCreate(&B.stream));
+            batch_reset(b);
+            tune_passes(B);
+            HIPCHK(hipStreamSynchronize(B.stream));
         } catch (...) {
             batch_release(b);
             delete b;
@@ -2563,6 +2655,26 @@ int prc_batch_correct(int n, prc_projector* const* projectors, const float* cons
     for (int v = 0; v < n && rc == PRC_OK; ++v) rc = prc_batch_estimate(b, v, host_out[v]);
     prc_batch_destroy(b);
     return rc;
+}
+
+int prc_tuning_report(char* json, size_t capacity) {
+    return guarded([&] {
+        std::string s = "[";
+        for (size_t i = 0; i < g_tuning.size(); ++i) {
+            const TuneRecord& r = g_tuning[i];
+            char buf[512];
+            snprintf(buf, sizeof(buf),
+                     "%s{\"volumes\": %d, \"weight_bytes\": %d, \"forward_items\": %d, \"adjoint_items\": %d, "
+                     "\"forward_ms\": {\"grid\": %.4f, \"queue\": %.4f}, \"adjoint_ms\": {\"grid\": %.4f, \"queue\": %.4f}, "
+                     "\"forward\": \"%s\", \"adjoint\": \"%s\"}",
+                     i ? ", " : "", r.vols, r.wbytes, r.items_fwd, r.items_adj, r.ms[0][0], r.ms[0][1], r.ms[1][0],
+                     r.ms[1][1], r.choice[0] ? "queue" : "grid", r.choice[1] ? "queue" : "grid");
+            s += buf;
+        }
+        s += "]";
+        if (!json || capacity < s.size() + 1) fail(PRC_ERR_INVALID, "report needs %zu bytes", s.size() + 1);
+        memcpy(json, s.c_str(), s.size() + 1);
+    });
 }
 
 int prc_profile_enable(int on) {
