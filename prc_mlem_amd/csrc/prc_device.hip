@@ -1102,8 +1102,8 @@ __device__ inline void load_offsets(const int* pl, int j, int count, int (&dl)[k
 // Shared body of both operators: acc += sum over the stored steps of one ray of
 // w[step][lane] * vec[step shift + lane], where `wrow` is the ray's first plane at lane 0,
 // `len` the plane stride, and vec0 + sign * D(step) the uniform gather base.
-template <class WT>
-__device__ inline float ray_planes(const WT* wrow, int len, int count, const int* pl, const float* vec0, int sign,
+template <class WT, class VT>
+__device__ inline float ray_planes(const WT* wrow, int len, int count, const int* pl, const VT* vec0, int sign,
                                    int lane, float acc) {
     for (int j = 0; j < count; j += kProjGroup) {
         int dl[kProjGroup];
@@ -1112,9 +1112,9 @@ __device__ inline float ray_planes(const WT* wrow, int len, int count, const int
 #pragma unroll
         for (int u = 0; u < kProjGroup; ++u) {
             const WT* wp = wrow + min(j + u, count - 1) * len;  // uniform plane base
-            const float* vp = vec0 + sign * dl[u];              // uniform gather base
+            const VT* vp = vec0 + sign * dl[u];                 // uniform gather base
             w[u] = WeightIO<WT>::load(wp + lane);
-            v[u] = gload(vp + lane);
+            v[u] = WeightIO<VT>::load(vp + lane);
         }
 #pragma unroll
         for (int u = 0; u < kProjGroup; ++u) {
@@ -1157,8 +1157,8 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
 }
 
 // out = A g where g already holds f / W (one item; see k_forward).
-template <class WT>
-__device__ inline void forward_item(const Vol& V, const int4 it, const float* __restrict__ gin,
+template <class WT, class VT>
+__device__ inline void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
                                     float* __restrict__ out, int lane) {
     const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
     const int row = it.y, x0 = it.z;
@@ -1171,7 +1171,7 @@ __device__ inline void forward_item(const Vol& V, const int4 it, const float* __
     const int* plin = V.plin;
     const int* kfirst = V.kfirst;
     const WT* tb = (const WT*)V.tbuf;
-    const float* g0 = gin + V.vbase + (size_t)row * nx + x0;
+    const VT* g0 = gin + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
         const RowInfo ti = gload(tinfo + p);
@@ -1187,29 +1187,30 @@ __device__ inline void forward_item(const Vol& V, const int4 it, const float* __
 // check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 // QUEUE selects the schedule: XCD-local work queues, or one wave per item in launch order.
-template <class WT, bool QUEUE>
+// The gathered vector is FP32, or FP16 in the mixed form (VT).
+template <class WT, class VT, bool QUEUE>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                                  int n_items, const float* __restrict__ gin,
+                                                  int n_items, const VT* __restrict__ gin,
                                                   float* __restrict__ out, int* __restrict__ queue) {
     const int lane = threadIdx.x & (kWave - 1);
     if (!QUEUE) {
         const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
-        if (it.w != 0) forward_item<WT>(vols[it.x], it, gin, out, lane);
+        if (it.w != 0) forward_item<WT, VT>(vols[it.x], it, gin, out, lane);
         return;
     }
     const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
-        if (it.w != 0) forward_item<WT>(vols[it.x], it, gin, out, lane);
+        if (it.w != 0) forward_item<WT, VT>(vols[it.x], it, gin, out, lane);
     }
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (one item; see k_adjoint).
-template <class WT>
-__device__ inline void adjoint_item(const Vol& V, const int4 it, const float* __restrict__ in,
+template <class WT, class VT>
+__device__ inline void adjoint_item(const Vol& V, const int4 it, const VT* __restrict__ in,
                                     float* __restrict__ out, int lane) {
     const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
     const int row = it.y, x0 = it.z;
@@ -1218,7 +1219,7 @@ __device__ inline void adjoint_item(const Vol& V, const int4 it, const float* __
     const int* kfirst = V.kfirst;
     const WT* wb = (const WT*)V.wbuf;
     const bool valid = lane < it.w;
-    const float* s0 = in + V.vbase + (size_t)row * nx + x0;
+    const VT* s0 = in + V.vbase + (size_t)row * nx + x0;
     float acc = 0.0f;
     for (int p = 0; p < R1; ++p) {
         const RowInfo ri = gload(rinfo + p);
@@ -1234,23 +1235,23 @@ __device__ inline void adjoint_item(const Vol& V, const int4 it, const float* __
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
 // planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
-template <class WT, bool QUEUE>
+template <class WT, class VT, bool QUEUE>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                                  int n_items, const float* __restrict__ in,
+                                                  int n_items, const VT* __restrict__ in,
                                                   float* __restrict__ out, int* __restrict__ queue) {
     const int lane = threadIdx.x & (kWave - 1);
     if (!QUEUE) {
         const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
-        if (it.w > 0) adjoint_item<WT>(vols[it.x], it, in, out, lane);
+        if (it.w > 0) adjoint_item<WT, VT>(vols[it.x], it, in, out, lane);
         return;
     }
     const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
-        if (it.w > 0) adjoint_item<WT>(vols[it.x], it, in, out, lane);
+        if (it.w > 0) adjoint_item<WT, VT>(vols[it.x], it, in, out, lane);
     }
 }
 
@@ -1269,30 +1270,40 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     (void)i;
 
 // g = f / W for the forward (ForwardOp::begin).
+// gh, when given, receives the FP16 copy that the mixed form's forward gathers.
 __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
-                         const float* __restrict__ f, float* __restrict__ g) {
+                         const float* __restrict__ f, float* __restrict__ g, __half* __restrict__ gh) {
     ITEM_PROLOGUE
-    g[i] = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
+    const float v = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
+    g[i] = v;
+    if (gh) gh[i] = __float2half_rn(v);
 }
 
 // Biggs-Andrews prediction y = max(f + lambda (f - prev), f / 2) (prc.cu:318-321), fused with
 // the forward scaling of y.
 __global__ void k_extrapolate(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
                               const Scal* __restrict__ sc, const float* __restrict__ f,
-                              const float* __restrict__ prev, float* __restrict__ y, float* __restrict__ g) {
+                              const float* __restrict__ prev, float* __restrict__ y, float* __restrict__ g,
+                              __half* __restrict__ gh) {
     ITEM_PROLOGUE
     const float lambda = sc[it.x].lambda;
     const float fi = f[i];
     const float yi = fmaxf(fmaf(lambda, fi - prev[i], fi), 0.5f * fi);
     y[i] = yi;
-    g[i] = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
+    const float v = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
+    g[i] = v;
+    if (gh) gh[i] = __float2half_rn(v);
 }
 
 // ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place.
+// rh, when given, receives the FP16 copy that the mixed form's adjoint gathers.
 __global__ void k_ratio(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
-                        const Scal* __restrict__ sc, const float* __restrict__ data, float* __restrict__ est) {
+                        const Scal* __restrict__ sc, const float* __restrict__ data, float* __restrict__ est,
+                        __half* __restrict__ rh) {
     ITEM_PROLOGUE
-    est[i] = gload(V.support + loc) ? data[i] / fmaxf(est[i], sc[it.x].eps) : 0.0f;
+    const float r = gload(V.support + loc) ? data[i] / fmaxf(est[i], sc[it.x].eps) : 0.0f;
+    est[i] = r;
+    if (rh) rh[i] = __float2half_rn(r);
 }
 
 // RDP MAP-EM surrogate update (prc.cu:279-309). Writes next and per-item FP64 partial sums.
@@ -1604,6 +1615,7 @@ struct prc_solver {
           *gf = nullptr;
     float* base[5] = {};  // f, prev, y, hprev, est as allocated (the loop rotates them)
     float* raw[7] = {};   // allocations: every vector has the projector's gather margin on both sides
+    __half* hraw[2] = {};  // mixed form: FP16 forward and adjoint inputs, same layout
     Scal* sc = nullptr;
     Scal sc0{};
     double* partial = nullptr;
@@ -1981,35 +1993,42 @@ void release(prc_projector* A) {
 enum Sched { kSchedGrid = 0, kSchedQueue = 1 };
 
 // `queue` holds kXcds counters in device memory owned by the caller.
-void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const float* g, float* out, int* queue,
-                 int sched, hipStream_t s) {
+// The gathered input is FP32, or FP16 (`half_in`, the mixed-form solver's iterates) with FP16
+// weights.
+template <class WT, class VT>
+void launch_forward(const Vol* dv, const int4* items, int n_items, const void* g, float* out, int* queue, int sched,
+                    unsigned nb, hipStream_t s) {
+    if (sched == kSchedQueue) k_forward<WT, VT, true><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)g, out, queue);
+    else k_forward<WT, VT, false><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)g, out, queue);
+}
+template <class WT, class VT>
+void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* in, float* out, int* queue, int sched,
+                    unsigned nb, hipStream_t s) {
+    if (sched == kSchedQueue) k_adjoint<WT, VT, true><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)in, out, queue);
+    else k_adjoint<WT, VT, false><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)in, out, queue);
+}
+
+void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const void* g, bool half_in, float* out,
+                 int* queue, int sched, hipStream_t s) {
     const unsigned all = blocks_for(n_items, kItemsPerBlock);
     const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
     if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("forward", s);
-    if (wbytes == 2) {
-        if (sched == kSchedQueue) k_forward<__half, true><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
-        else k_forward<__half, false><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
-    } else {
-        if (sched == kSchedQueue) k_forward<float, true><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
-        else k_forward<float, false><<<nb, 256, 0, s>>>(dv, items, n_items, g, out, queue);
-    }
+    if (wbytes == 4) launch_forward<float, float>(dv, items, n_items, g, out, queue, sched, nb, s);
+    else if (half_in) launch_forward<__half, __half>(dv, items, n_items, g, out, queue, sched, nb, s);
+    else launch_forward<__half, float>(dv, items, n_items, g, out, queue, sched, nb, s);
     HIPCHK(hipGetLastError());
 }
 
-void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const float* in, float* out, int* queue,
-                 int sched, hipStream_t s) {
+void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const void* in, bool half_in, float* out,
+                 int* queue, int sched, hipStream_t s) {
     const unsigned all = blocks_for(n_items, kItemsPerBlock);
     const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
     if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("adjoint", s);
-    if (wbytes == 2) {
-        if (sched == kSchedQueue) k_adjoint<__half, true><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-        else k_adjoint<__half, false><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    } else {
-        if (sched == kSchedQueue) k_adjoint<float, true><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-        else k_adjoint<float, false><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    }
+    if (wbytes == 4) launch_adjoint<float, float>(dv, items, n_items, in, out, queue, sched, nb, s);
+    else if (half_in) launch_adjoint<__half, __half>(dv, items, n_items, in, out, queue, sched, nb, s);
+    else launch_adjoint<__half, float>(dv, items, n_items, in, out, queue, sched, nb, s);
     HIPCHK(hipGetLastError());
 }
 
@@ -2040,6 +2059,7 @@ struct Batch {
     const int* item_begin = nullptr;  // per-volume ranges of items_all
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
+    __half *gh = nullptr, *rh = nullptr;  // mixed form: FP16 copies of the forward and adjoint inputs
     Scal* sc = nullptr;
     double* partial = nullptr;
     bool accel = false, any_prior = false;
@@ -2063,18 +2083,20 @@ void batch_iteration(const Batch& B) {
     {
         ProfScope ps("extrapolate", B.stream);
         if (B.accel) {
-            k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf);
+            k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf, B.gh);
             start = B.y;
         } else {
-            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf);
+            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf, B.gh);
         }
     }
-    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, B.fwd_sched, B.stream);
+    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
+                B.fwd_sched, B.stream);
     {
         ProfScope ps("ratio", B.stream);
-        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est);
+        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, B.rh);
     }
-    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, B.adj_sched, B.stream);
+    run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev,
+                B.queue, B.adj_sched, B.stream);
     {
         ProfScope ps("update", B.stream);
         k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
@@ -2114,8 +2136,12 @@ void tune_passes(Batch& B) {
         for (int op = 0; op < 2; ++op)
             for (int sched = 0; sched < 2; ++sched) {
                 auto launch = [&] {
-                    if (op == 0) run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gf, B.est, B.queue, sched, B.stream);
-                    else run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.est, B.prev, B.queue, sched, B.stream);
+                    if (op == 0)
+                        run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf,
+                                    B.gh != nullptr, B.est, B.queue, sched, B.stream);
+                    else
+                        run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est,
+                                    B.rh != nullptr, B.prev, B.queue, sched, B.stream);
                 };
                 launch();  // warm-up
                 HIPCHK(hipEventRecord(e0, B.stream));
@@ -2336,10 +2362,10 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
         float* g = A->scratch_in + A->margin;  // f / W with the forward's gather margin
         {
             ProfScope ps("extrapolate", 0);
-            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g);
+            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, dev_out, A->queue, A->fwd_sched, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
     });
 }
 
@@ -2352,7 +2378,7 @@ int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out)
         }
         float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
         HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, dev_out, A->queue, A->adj_sched, 0);
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, false, dev_out, A->queue, A->adj_sched, 0);
     });
 }
 
@@ -2431,6 +2457,11 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
                 HIPCHK(hipMemset(S->raw[i], 0, padded * sizeof(float)));
             }
             S->g = S->raw[5] + A->margin, S->gf = S->raw[6] + A->margin;
+            if (A->wbytes == 2)
+                for (int i = 0; i < 2; ++i) {
+                    S->hraw[i] = dalloc<__half>(padded);
+                    HIPCHK(hipMemset(S->hraw[i], 0, padded * sizeof(__half)));
+                }
             for (int i = 0; i < 5; ++i) S->base[i] = S->raw[i] + A->margin;
             HIPCHK(hipMemcpy(S->g, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
             S->sc = dalloc<Scal>(1);
@@ -2470,6 +2501,7 @@ int prc_solver_reset(prc_solver* S) {
 void prc_solver_destroy(prc_solver* S) {
     if (!S) return;
     for (float*& b : S->raw) dfree(b);
+    for (__half*& b : S->hraw) dfree(b);
     dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
     delete S;
 }
@@ -2482,6 +2514,7 @@ static Batch solver_batch(prc_solver* S) {
     B.fwd_sched = A->fwd_sched, B.adj_sched = A->adj_sched;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
+    if (S->hraw[0]) B.gh = S->hraw[0] + S->A->margin, B.rh = S->hraw[1] + S->A->margin;
     B.sc = S->sc, B.partial = S->partial;
     B.accel = S->p.accelerate != 0, B.any_prior = S->p.beta > 0.0f;
     return B;
@@ -2601,6 +2634,12 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             B.g = (float*)own(dalloc<float>(b->total));
             B.gf = (float*)own(dalloc<float>(b->total));
             HIPCHK(hipMemset(B.gf, 0, b->total * sizeof(float)));
+            if (wbytes == 2) {
+                B.gh = (__half*)own(dalloc<__half>(b->total));
+                B.rh = (__half*)own(dalloc<__half>(b->total));
+                HIPCHK(hipMemset(B.gh, 0, b->total * sizeof(__half)));
+                HIPCHK(hipMemset(B.rh, 0, b->total * sizeof(__half)));
+            }
             for (int i = 0; i < 5; ++i) b->base[i] = (float*)own(dalloc<float>(b->total));
             HIPCHK(hipMemcpy(B.g, g.data(), b->total * sizeof(float), hipMemcpyHostToDevice));
             B.sc = (Scal*)own(dalloc<Scal>(n));
