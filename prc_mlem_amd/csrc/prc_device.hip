@@ -1023,10 +1023,11 @@ __global__ void k_forward_bounds(Vol V, const int* kglob) {
     const int row = unit / nseg, seg = unit % nseg;
     const int tw = segment_width(gload(V.row_len + row), seg);
     int kt = 0;
-    if (tw > 0) {
+    if (tw > 0 && p > 0) {  // the near block (p = 0) always holds its 125 bins
         const int tx0 = gload(V.row_xa + row) + kWave * seg, y = row % V.ny, z = row / V.ny;
         const int kf = gload(V.kfirst + p), kend = gload(kglob + p);
-        for (int k = kf; k < kend; ++k) {
+        // Downward from the ray's global bound: the first step that reaches this segment is the bound.
+        for (int k = kend - 1; k >= kf && kt == 0; --k) {
             const int4 d = gload(V.pstep + (size_t)p * V.LP + k);
             const int ys = y - d.y, zs = z - d.z;
             if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
