@@ -51,7 +51,7 @@ def timed(fn, steps, warmup, sync):
     return time.perf_counter() - t0
 
 
-def measured_traffic(kernel, form):
+def measured_traffic(kernel, form, sched=None):
     """HBM bytes per launch of `kernel` on this workload from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
     plain copy of known size), or None when no summary covers it."""
@@ -62,7 +62,8 @@ def measured_traffic(kernel, form):
         table = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, ValueError, IndexError):
         return None
-    entry = table.get("kernels", {}).get(f"{kernel}/{form}")
+    key = f"{kernel}/{form}" + (f"/{sched}" if sched else "")
+    entry = table.get("kernels", {}).get(key)
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
@@ -202,8 +203,9 @@ def main():
         wbytes = st0["weight_entries"] * b_w * vols     # source-major planes (the adjoint's stream)
         wbytes_t = st0["target_entries"] * b_w * vols   # target-major planes (the forward's stream)
         n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
-        per_launch = {"forward": wbytes_t + vols * (4 * n + 4 * n),
-                      "adjoint": wbytes + vols * (4 * n + 8 * n_act + n)}
+        b_v = 2 if mixed is not None else 4  # the mixed solver gathers FP16 copies of its iterates
+        per_launch = {"forward": wbytes_t + vols * (b_v * n + 4 * n),
+                      "adjoint": wbytes + vols * (b_v * n + 8 * n_act + n)}
         if name in per_launch:
             mean_s = k[name]["mean_ms"] / 1e3
             achieved = per_launch[name] / mean_s / 1e9
@@ -231,8 +233,15 @@ def main():
                              "effective_GBps": round(vrs_call * b_step / t / 1e9, 1),
                              "effective_frac_of_hbm_peak": round(vrs_call * b_step / t / 1e9 / HBM_PEAK_GBS, 3)}
         projector[form] = entry
+    tuning = prc.tuning_report()
     if roofline is not None:
-        roofline["traffic"] = measured_traffic(roofline["kernel"], "mixed" if mixed is not None else "fp32")
+        form = "mixed" if mixed is not None else "fp32"
+        sched = None
+        if roofline["kernel"] in ("forward", "adjoint"):
+            mine = [t for t in tuning if t["weight_bytes"] == (2 if mixed is not None else 4)]
+            sched = mine[-1][roofline["kernel"]] if mine else None
+            roofline["schedule"] = sched
+        roofline["traffic"] = measured_traffic(roofline["kernel"], form, sched)
 
     cpu = None
     if args.cpu_slices > 0 and fp32 is not None:
@@ -264,7 +273,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "projector_effective": projector,
     }
     out.update(result)
-    out["projector_tuning"] = prc.tuning_report()
+    out["projector_tuning"] = tuning
     if fp32 is not None:
         out["fp32"] = fp32
     if mixed is not None:
