@@ -83,6 +83,29 @@ def cpu_baseline(rho, data, slices, seconds_budget):
     return t, near + far, oracle.threads()
 
 
+def accuracy_vs_fp64(rho, data, crop):
+    """NRMSE, 3D SSIM and count error of the GPU corrections (plain FP32 and mixed) against the FP64
+    oracle on a central crop of the workload (the FP64 CPU oracle evaluates the range model exactly
+    at every step; the full volume would take it many minutes)."""
+    from oracle import oracle
+    from prc_mlem_amd.evaluate import Evaluator
+    sl = tuple(slice(n // 2 - c // 2, n // 2 - c // 2 + c) for n, c in zip(SHAPE, crop))
+    r = np.ascontiguousarray(np.maximum(rho[sl], np.float32(1e-3)))
+    g = np.ascontiguousarray(np.where(r >= np.float32(0.05), np.maximum(data[sl], 0), 0).astype(np.float32))
+    t0 = time.perf_counter()
+    ref = oracle.Projector(r, SPACING, ISOTOPE, RAYS, prec=64).solve(g, ITERS, BETA, GAMMA, True).reshape(r.shape)
+    ev = Evaluator(np.maximum(ref, 0))
+    out = {"region": "central crop %dx%dx%d (NZ NY NX) of the workload" % tuple(crop), "reference": "FP64 oracle",
+           "oracle_seconds": round(time.perf_counter() - t0, 1)}
+    for form in ("fp32", "mixed"):
+        A = prc.Projector(r, SPACING, ISOTOPE, RAYS, precision=form)
+        est = np.asarray(prc.correct(A, g, ITERS, BETA, GAMMA, True)).reshape(r.shape)
+        A.close()
+        m = ev(est)
+        out[form] = {"nrmse": m["nrmse"], "ssim": m["ssim"], "count_error": m["count_bias"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +115,8 @@ def main():
     ap.add_argument("--only", choices=["both", "mixed", "fp32"], default="both")
     ap.add_argument("--cpu-slices", type=int, default=48, help="Z slices of the CPU-baseline slab (0 = skip)")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--accuracy-crop", type=int, nargs=3, default=[16, 64, 64], metavar=("NZ", "NY", "NX"),
+                    help="central crop on which both corrections are compared with the FP64 oracle (0 0 0 = skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes three times (the known-size kernel that "
                          "scales the HBM byte counters of a profiler run) and exit")
@@ -258,6 +283,11 @@ def main():
                          f"({t_slab:.1f} s, {vrs_slab:.3g} voxel-ray steps); vols/s extrapolated from its "
                          f"{rate:.3g} steps/s to the full volume's {vrs_full:.3g} steps"}
 
+    accuracy = None
+    if min(args.accuracy_crop) > 0 and args.cpu_slices > 0:
+        log(f"accuracy: GPU corrections vs the FP64 oracle on a {args.accuracy_crop} crop")
+        accuracy = accuracy_vs_fp64(rho, datas[0], args.accuracy_crop)
+
     out = {
         "metric": "corrected vols/s + projector GB/s, 128x128x159 68Ga: mixed+batched vs FP32",
         "value": round(value, 4), "unit": "corrected vols/s", "n_gpus": 1, "steps": args.steps,
@@ -273,6 +303,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "projector_effective": projector,
     }
     out.update(result)
+    out["accuracy_vs_fp64"] = accuracy
     out["projector_tuning"] = tuning
     if fp32 is not None:
         out["fp32"] = fp32
