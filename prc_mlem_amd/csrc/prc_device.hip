@@ -1586,6 +1586,11 @@ struct prc_projector {
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
     int* queue = nullptr;       // work-queue counters of the projector passes
+    // The normalisation runs on a stream of its own that does not wait for other streams, so
+    // that it can overlap a batch that is iterating on other projectors. It first waits for the
+    // null stream (solvers) and for the streams of the batches that use this projector.
+    hipStream_t stream = nullptr;
+    std::vector<hipStream_t> users;
     int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
     bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
@@ -1687,13 +1692,13 @@ void launch_far(prc_projector* A, int pass) {
     const bool exact = A->hv.exact;
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
     if (pass == 1) {
-        ProfScope ps("far_count", 0);
-        if (exact) k_far<1, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
-        else k_far<1, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
+        ProfScope ps("far_count", A->stream);
+        if (exact) k_far<1, true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
+        else k_far<1, false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
     } else {
-        ProfScope ps("far_fill", 0);
-        if (exact) k_far<2, true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
-        else k_far<2, false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
+        ProfScope ps("far_fill", A->stream);
+        if (exact) k_far<2, true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
+        else k_far<2, false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1701,16 +1706,20 @@ void launch_far(prc_projector* A, int pass) {
 template <class WT>
 void launch_near(prc_projector* A) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
-    ProfScope ps("near", 0);
+    ProfScope ps("near", A->stream);
     if (A->memo) {  // march the listed sources, then copy the templates to the uniform ones
         const unsigned nl = blocks_for(A->n_list, 256);
-        if (A->hv.exact) k_near_list<true, WT><<<nl, 256>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum, A->counters);
-        else k_near_list<false, WT><<<nl, 256>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum, A->counters);
-        k_near_fill<WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
+        if (A->hv.exact)
+            k_near_list<true, WT><<<nl, 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum,
+                                                             A->counters);
+        else
+            k_near_list<false, WT><<<nl, 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum,
+                                                              A->counters);
+        k_near_fill<WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
     } else if (A->hv.exact) {
-        k_near<true, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+        k_near<true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters);
     } else {
-        k_near<false, WT><<<nb, 256>>>(A->dv, A->items_act, A->n_act, A->counters);
+        k_near<false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1845,41 +1854,45 @@ void normalise_projector(prc_projector* A) {
     Vol& V = A->hv;
     const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1;
-    HIPCHK(hipMemsetAsync(A->rinfo, 0, units * R1 * sizeof(RowInfo), 0));
-    HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), 0));
-    HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), 0));
+    hipStream_t st = A->stream;
+    HIPCHK(hipStreamSynchronize(nullptr));  // solver work on this projector
+    for (hipStream_t u : A->users) HIPCHK(hipStreamSynchronize(u));  // batches on this projector
+    HIPCHK(hipMemsetAsync(A->rinfo, 0, units * R1 * sizeof(RowInfo), st));
+    HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), st));
+    HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), st));
     if (A->cls) {
-        ProfScope ps("near_classify", 0);
+        ProfScope ps("near_classify", st);
         int* keys = A->memo_meta;
-        HIPCHK(hipMemsetAsync(keys, 0xff, kMaxTemplates * sizeof(int), 0));
-        HIPCHK(hipMemsetAsync(keys + kMaxTemplates, 0x7f, kMaxTemplates * sizeof(int), 0));
-        HIPCHK(hipMemsetAsync(keys + 2 * kMaxTemplates, 0, 2 * sizeof(int), 0));
+        HIPCHK(hipMemsetAsync(keys, 0xff, kMaxTemplates * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(keys + kMaxTemplates, 0x7f, kMaxTemplates * sizeof(int), st));
+        HIPCHK(hipMemsetAsync(keys + 2 * kMaxTemplates, 0, 2 * sizeof(int), st));
         const unsigned nb = blocks_for((long long)A->n, 256);
-        k_classify<<<nb, 256>>>(A->dv, A->cls, (unsigned*)keys, keys + kMaxTemplates);
-        k_memo_list<<<nb, 256>>>((int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
+        k_classify<<<nb, 256, 0, st>>>(A->dv, A->cls, (unsigned*)keys, keys + kMaxTemplates);
+        k_memo_list<<<nb, 256, 0, st>>>((int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
         HIPCHK(hipGetLastError());
     }
     if (A->wbytes == 2) launch_far<__half>(A, 1);
     else launch_far<float>(A, 1);
     {
-        ProfScope ps("layout", 0);
-        k_row_totals<<<blocks_for((long long)units, 256), 256>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)units, A->total);
-        k_row_offsets<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->row_base);
-        k_kglob<<<R1, 256>>>(V, A->kglob);
-        k_forward_bounds<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->kglob);
-        k_target_totals<<<blocks_for((long long)units, 256), 256>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024>>>(A->row_tot, A->row_base, (int)units, A->total + 1);
-        k_target_offsets<<<blocks_for((long long)units * R1, 256), 256>>>(V, A->row_base);
+        ProfScope ps("layout", st);
+        k_row_totals<<<blocks_for((long long)units, 256), 256, 0, st>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024, 0, st>>>(A->row_tot, A->row_base, (int)units, A->total);
+        k_row_offsets<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->row_base);
+        k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
+        k_forward_bounds<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->kglob);
+        k_target_totals<<<blocks_for((long long)units, 256), 256, 0, st>>>(V, A->row_tot);
+        k_scan_rows<<<1, 1024, 0, st>>>(A->row_tot, A->row_base, (int)units, A->total + 1);
+        k_target_offsets<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->row_base);
         HIPCHK(hipGetLastError());
     }
     long long total[2] = {0, 0};
-    HIPCHK(hipMemcpy(total, A->total, sizeof(total), hipMemcpyDeviceToHost));
     std::vector<int> kglob(R1);
-    HIPCHK(hipMemcpy(kglob.data(), A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost));
+    int ctr[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(total, A->total, sizeof(total), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(kglob.data(), A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (A->cls) HIPCHK(hipMemcpyAsync(ctr, A->memo_meta + 2 * kMaxTemplates, sizeof(ctr), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     if (A->cls) {
-        int ctr[2];
-        HIPCHK(hipMemcpy(ctr, A->memo_meta + 2 * kMaxTemplates, sizeof(ctr), hipMemcpyDeviceToHost));
         A->n_list = ctr[0];
         A->stats.near_uniform = ctr[1];
         // Worth a second pass only when a good share of the sources is uniform.
@@ -1897,8 +1910,8 @@ void normalise_projector(prc_projector* A) {
     A->stats.target_entries = total[1];
     A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
     V.wbuf = A->wbuf, V.tbuf = A->tbuf;
-    HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
-    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, 0));
+    HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
     if (A->wbytes == 2) {
         launch_near<__half>(A);
         launch_far<__half>(A, 2);
@@ -1906,6 +1919,7 @@ void normalise_projector(prc_projector* A) {
         launch_near<float>(A);
         launch_far<float>(A, 2);
     }
+    HIPCHK(hipStreamSynchronize(st));
 }
 
 // Host replay of the layout and of every index range the forward and adjoint read (PRC_AUDIT=1):
@@ -1979,6 +1993,7 @@ void read_counters(prc_projector* A) {
 }
 
 void release(prc_projector* A) {
+    if (A->stream) (void)hipStreamSynchronize(A->stream), (void)hipStreamDestroy(A->stream), A->stream = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
@@ -2225,6 +2240,11 @@ struct prc_batch {
 namespace {
 
 void batch_release(prc_batch* b) {
+    if (b->B.stream) {
+        (void)hipStreamSynchronize(b->B.stream);
+        for (prc_projector* A : b->A)
+            if (A) A->users.erase(std::remove(A->users.begin(), A->users.end(), b->B.stream), A->users.end());
+    }
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->B.stream) (void)hipStreamDestroy(b->B.stream);
@@ -2325,7 +2345,9 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
         A->g = *geometry;
         try {
             const double t0 = now_s();
+            HIPCHK(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
             setup_projector(A, host_rho, host_table);
+            HIPCHK(hipDeviceSynchronize());  // setup ran on the null stream
             normalise_projector(A);
             HIPCHK(hipDeviceSynchronize());
             A->stats.norm_seconds = now_s() - t0;
@@ -2645,7 +2667,8 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             HIPCHK(hipMemcpy(B.g, g.data(), b->total * sizeof(float), hipMemcpyHostToDevice));
             B.sc = (Scal*)own(dalloc<Scal>(n));
             B.partial = (double*)own(dalloc<double>(2 * all.size()));
-            HIPCHK(hipStreamCreate(&B.stream));
+            HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+            for (int v = 0; v < n; ++v) projectors[v]->users.push_back(B.stream);
             batch_reset(b);
             tune_passes(B);
             HIPCHK(hipStreamSynchronize(B.stream));
