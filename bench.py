@@ -194,8 +194,7 @@ def main():
         turn = [0]
 
         def step_set(k):
-            for a in As[k]:
-                a.normalise()
+            prc.normalise_all(As[k])  # the batch's projectors build concurrently, one stream each
             Bs[k].run(ITERS)
 
         def step_seq():

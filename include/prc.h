@@ -111,6 +111,11 @@ void prc_projector_destroy(prc_projector* projector);
  * the march still queued. For callers that time or repeat the build without re-uploading. */
 int prc_projector_normalise(prc_projector* projector);
 
+/* Normalisation of several projectors at once, each on its own stream so that their kernels
+ * overlap; same result as prc_projector_normalise on each. Returns when all are done. No
+ * counterpart in the reference (one projector per run); for batches of volumes. */
+int prc_projectors_normalise(int n, prc_projector* const* projectors);
+
 /* dev_out = A dev_in, overwriting every voxel (Projector::forward, prc.cu:430-434). */
 int prc_projector_forward(prc_projector* projector, const float* dev_in, float* dev_out);
 /* dev_out = A^T dev_in on active voxels; other voxels are left untouched

@@ -67,6 +67,7 @@ SIGNATURES = {
     "prc_projector_create": (C.c_int, [C.POINTER(Geometry), _F32P, C.c_void_p, C.POINTER(C.c_void_p)]),
     "prc_projector_destroy": (None, [C.c_void_p]),
     "prc_projector_normalise": (C.c_int, [C.c_void_p]),
+    "prc_projectors_normalise": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "prc_projector_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_forward_host": (C.c_int, [C.c_void_p, _F32P, _F32P]),
@@ -338,6 +339,12 @@ def profile_report():
     buf = C.create_string_buffer(1 << 16)
     _check(lib().prc_profile_report(buf, len(buf)))
     return json.loads(buf.value.decode())
+
+
+def normalise_all(projectors):
+    """Normalise several projectors at once (each on its own stream; their kernels overlap)."""
+    arr = (C.c_void_p * len(projectors))(*[p.handle for p in projectors])
+    _check(lib().prc_projectors_normalise(len(projectors), arr))
 
 
 def tuning_report():

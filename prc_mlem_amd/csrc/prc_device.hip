@@ -1591,6 +1591,9 @@ struct prc_projector {
     // null stream (solvers) and for the streams of the batches that use this projector.
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> users;
+    // Pinned staging of the sizes normalise_begin reads back (so the read-back does not block
+    // the host): total[2] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
+    void* h_stage = nullptr;
     int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
     bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
@@ -1850,7 +1853,9 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
 // The normalisation pass (Projector::Projector, prc.cu:405-422): count, lay out both copies of
 // the matrix, then march again writing every weight (source-major for the adjoint, target-major
 // for the forward) and W. One host synchronisation (the buffer sizes); buffers are reused.
-void normalise_projector(prc_projector* A) {
+// Phase 1, enqueued on the projector's stream: counts, layout, and the read-back of the sizes into
+// the projector's pinned staging (normalise_finish reads them after a stream sync).
+void normalise_begin(prc_projector* A) {
     Vol& V = A->hv;
     const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1;
@@ -1885,13 +1890,23 @@ void normalise_projector(prc_projector* A) {
         k_target_offsets<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->row_base);
         HIPCHK(hipGetLastError());
     }
-    long long total[2] = {0, 0};
-    std::vector<int> kglob(R1);
-    int ctr[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(total, A->total, sizeof(total), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(kglob.data(), A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost, st));
-    if (A->cls) HIPCHK(hipMemcpyAsync(ctr, A->memo_meta + 2 * kMaxTemplates, sizeof(ctr), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if (!A->h_stage) HIPCHK(hipHostMalloc(&A->h_stage, 2 * sizeof(long long) + (2 + R1) * sizeof(int)));
+    long long* total = (long long*)A->h_stage;
+    int* ctr = (int*)(total + 2);
+    HIPCHK(hipMemcpyAsync(total, A->total, 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ctr + 2, A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost, st));
+    if (A->cls) HIPCHK(hipMemcpyAsync(ctr, A->memo_meta + 2 * kMaxTemplates, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+}
+
+// Phase 2, after phase 1 has completed: buffers, then the near block and the fill, enqueued on the
+// projector's stream.
+void normalise_finish(prc_projector* A) {
+    Vol& V = A->hv;
+    hipStream_t st = A->stream;
+    const int R1 = V.n_rays + 1;
+    const long long* total = (const long long*)A->h_stage;
+    const int* ctr = (const int*)(total + 2);
+    const std::vector<int> kglob(ctr + 2, ctr + 2 + R1);
     if (A->cls) {
         A->n_list = ctr[0];
         A->stats.near_uniform = ctr[1];
@@ -1919,8 +1934,18 @@ void normalise_projector(prc_projector* A) {
         launch_near<float>(A);
         launch_far<float>(A, 2);
     }
-    HIPCHK(hipStreamSynchronize(st));
 }
+
+// The normalisation pass (Projector::Projector, prc.cu:405-422) of several projectors at once:
+// each runs on its own stream, so their kernels fill each other's tails. Returns when all are done.
+void normalise_projectors(prc_projector* const* As, int n) {
+    for (int i = 0; i < n; ++i) normalise_begin(As[i]);
+    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
+    for (int i = 0; i < n; ++i) normalise_finish(As[i]);
+    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
+}
+
+void normalise_projector(prc_projector* A) { normalise_projectors(&A, 1); }
 
 // Host replay of the layout and of every index range the forward and adjoint read (PRC_AUDIT=1):
 // each (segment, ray) run must follow the previous one in the buffer, plane reads must stay
@@ -1994,6 +2019,7 @@ void read_counters(prc_projector* A) {
 
 void release(prc_projector* A) {
     if (A->stream) (void)hipStreamSynchronize(A->stream), (void)hipStreamDestroy(A->stream), A->stream = nullptr;
+    if (A->h_stage) (void)hipHostFree(A->h_stage), A->h_stage = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
@@ -2366,6 +2392,18 @@ int prc_projector_normalise(prc_projector* A) {
     return guarded([&] {
         if (!A) fail(PRC_ERR_INVALID, "null argument");
         normalise_projector(A);
+    });
+}
+
+int prc_projectors_normalise(int n, prc_projector* const* projectors) {
+    return guarded([&] {
+        if (n <= 0 || !projectors) fail(PRC_ERR_INVALID, "invalid argument");
+        for (int i = 0; i < n; ++i) {
+            if (!projectors[i]) fail(PRC_ERR_INVALID, "null projector");
+            for (int j = 0; j < i; ++j)
+                if (projectors[j] == projectors[i]) fail(PRC_ERR_INVALID, "a projector is listed twice");
+        }
+        normalise_projectors(projectors, n);
     });
 }
 

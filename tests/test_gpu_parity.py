@@ -227,3 +227,20 @@ def test_uniform_neighbourhood_memo_matches_oracle(prc, oracle_mod, blocks):
     assert rel_l2(A.forward(x), O.forward(x)) <= 1e-6
     assert rel_l2(A.adjoint(y), O.adjoint(y)) <= 1e-6
     A.close()
+
+
+def test_concurrent_normalisation_matches_single(prc, phantom, blocks):
+    # Several projectors normalised together (one stream each) give the single builds' weights.
+    rho, x, y, sup = phantom
+    rho2, x2, _ = blocks
+    announce("concurrent normalisation of three projectors")
+    P = [prc.Projector(rho, SPACING, "68Ga", 32), prc.Projector(rho2, SPACING, "18F", 16, precision="mixed"),
+         prc.Projector(rho, (0.9, 1.2, 1.5), "124I", 24)]
+    ref = [(p.W, p.forward(v), p.adjoint(v)) for p, v in zip(P, (x, x2, y))]
+    prc.normalise_all(P)
+    for p, v, (w, f, a) in zip(P, (x, x2, y), ref):
+        np.testing.assert_array_equal(p.W, w)
+        np.testing.assert_array_equal(p.forward(v), f)
+        np.testing.assert_array_equal(p.adjoint(v), a)
+    for p in P:
+        p.close()
