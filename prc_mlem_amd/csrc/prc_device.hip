@@ -125,6 +125,14 @@ struct RowInfo {        // one per (segment, ray); 32 bytes, one scalar load
     int xa, len;        // the segment's x range [xa, xa + len) = plane width (len 0: empty segment)
     int pad0, pad1;
 };
+// The planes of one segment in one copy of the matrix: the runs of all its rays, in ray order
+// (near block first), follow each other with no gap, so they form one sequence of planes of the
+// segment's width; the shift list holds the gather shift of each of those planes.
+struct PlaneRun {       // 16 bytes, one scalar load
+    long long off;      // first entry of the segment's planes in the weight buffer
+    int planes;         // planes of all its rays
+    int dbase;          // first entry of its shifts in the shift list
+};
 struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
     int dp;             // padded linear offset of the step's voxel from the source
@@ -178,6 +186,10 @@ struct Vol {            // device descriptor of one volume
     RowInfo* tinfo;     // [ny*nz*nseg][n_rays + 1] target-major layout (planes over target x)
     void* wbuf;         // source-major weights (float or __half): the adjoint's stream
     void* tbuf;         // target-major weights: the forward's stream
+    const PlaneRun* sruns;  // [ny*nz*nseg] source-major plane sequence of each segment
+    const PlaneRun* truns;  // [ny*nz*nseg] target-major plane sequence of each segment
+    const int* slist;       // source-major shifts +D (the adjoint gathers at s + D)
+    const int* tlist;       // target-major shifts -D (the forward gathers at t - D)
     const FarStep* fstep;   // [n_rays][L]
     const int4* pstep;      // [n_rays + 1][LP] (dx, dy, dz) of each step; p = 0 is the near block
     const int* plin;        // [n_rays + 1][LP] linear voxel offset (dz*ny + dy)*nx + dx of each step
@@ -940,19 +952,27 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
 }
 
 // Plane offsets of the segment layout, computed on the device after the counting pass: the near
-// block = 125 planes for every non-empty segment; empty segments hold nothing.
+// block = 125 planes for every non-empty segment; empty segments hold nothing. The runs of a
+// segment follow each other without a gap, and every segment starts on a 64-entry boundary, so a
+// full segment's planes are whole cache lines.
 __device__ inline int segment_width(int len, int seg) { return min(kWave, max(0, len - kWave * seg)); }
-// Every (segment, ray) run starts on a 64-entry boundary, so a full segment's planes are whole
-// cache lines; only runs of partial segments are padded (at most 63 entries each).
-__host__ __device__ inline long long run_size(int cnt, int w) { return ((long long)cnt * w + kWave - 1) / kWave * kWave; }
+__host__ __device__ inline long long run_size(int cnt, int w) { return (long long)cnt * w; }
+__host__ __device__ inline long long align_up(long long v, long long a) { return (v + a - 1) / a * a; }
+// Shift lists: each segment's list starts on a multiple of kListAlign entries, and kListSlack zero
+// entries follow the last list (reads of a segment's last plane group may run past its end).
+constexpr int kListAlign = 4;
+constexpr int kListSlack = 64;
 
-__global__ void k_row_totals(Vol V, long long* unit_tot) {
+// Per segment: the runs' offsets relative to the segment, its total entries (aligned) and its
+// planes (the list length, aligned).
+__global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
     const int unit = blockIdx.x * blockDim.x + threadIdx.x;
     if (unit >= units) return;
     const int row = unit / V.nseg, seg = unit % V.nseg;
     const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
     long long off = 0;
+    int planes = 0;
     for (int p = 0; p < R1; ++p) {
         RowInfo* e = V.rinfo + (size_t)unit * R1 + p;
         const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
@@ -961,41 +981,53 @@ __global__ void k_row_totals(Vol V, long long* unit_tot) {
         e->xa = x0;
         e->len = w;
         off += run_size(cnt, w);
+        planes += cnt;
     }
-    unit_tot[unit] = off;
+    unit_tot[unit] = align_up(off, kWave);
+    unit_planes[unit] = align_up(planes, kListAlign);
+    runs[unit].planes = planes;
 }
 
-// Exclusive prefix sum of the row totals by one block in a fixed order; *total = grand total.
-__global__ __launch_bounds__(1024) void k_scan_rows(const long long* __restrict__ row_tot,
-                                                      long long* __restrict__ row_base, int rows,
-                                                      long long* __restrict__ total) {
-    __shared__ long long part[1024];
-    const int t = threadIdx.x, nt = blockDim.x;
-    const int chunk = (rows + nt - 1) / nt;
-    const int b = min(rows, t * chunk), e = min(rows, b + chunk);
-    long long s = 0;
-    for (int i = b; i < e; ++i) s += row_tot[i];
-    part[t] = s;
+// Exclusive prefix sums of the per-segment entries and list lengths by one block, 1024 segments at
+// a time in a fixed order; total_a and total_b receive the grand totals.
+__global__ __launch_bounds__(1024) void k_scan_units(const long long* __restrict__ a, const long long* __restrict__ b,
+                                                       long long* __restrict__ a_base, long long* __restrict__ b_base,
+                                                       int n, long long* __restrict__ total_a,
+                                                       long long* __restrict__ total_b) {
+    __shared__ long long wa[16], wb[16];
+    __shared__ long long carry[2];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+    if (t == 0) carry[0] = carry[1] = 0;
     __syncthreads();
-    for (int o = 1; o < nt; o <<= 1) {  // inclusive Hillis-Steele scan of the chunk sums
-        const long long v = t >= o ? part[t - o] : 0;
+    for (int base = 0; base < n; base += 1024) {
+        const int i = base + t;
+        const long long va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
+        long long sa = va, sb = vb;  // inclusive scan within the wave
+        for (int o = 1; o < kWave; o <<= 1) {
+            const long long xa = __shfl_up(sa, o), xb = __shfl_up(sb, o);
+            if (lane >= o) sa += xa, sb += xb;
+        }
+        if (lane == kWave - 1) wa[wv] = sa, wb[wv] = sb;
         __syncthreads();
-        part[t] += v;
+        long long pa = carry[0], pb = carry[1];
+        for (int k = 0; k < wv; ++k) pa += wa[k], pb += wb[k];
+        if (i < n) a_base[i] = pa + sa - va, b_base[i] = pb + sb - vb;
+        __syncthreads();
+        if (t == 1023) carry[0] = pa + sa, carry[1] = pb + sb;  // values past n are zeros
         __syncthreads();
     }
-    long long base = t ? part[t - 1] : 0;
-    for (int i = b; i < e; ++i) {
-        row_base[i] = base;
-        base += row_tot[i];
-    }
-    if (t == nt - 1) *total = part[nt - 1];
+    if (t == 0) *total_a = carry[0], *total_b = carry[1];
 }
 
-__global__ void k_row_offsets(Vol V, const long long* __restrict__ unit_base) {
+// Absolute offsets: every run moves by its segment's base; the segment's plane sequence and list.
+__global__ void k_row_offsets(Vol V, const long long* __restrict__ unit_base, const long long* __restrict__ list_base,
+                              PlaneRun* runs) {
     const int R1 = V.n_rays + 1;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)V.ny * V.nz * V.nseg * R1) return;
-    V.rinfo[id].off += unit_base[id / R1];
+    const long long unit = id / R1;
+    V.rinfo[id].off += unit_base[unit];
+    if (id % R1 == 0) runs[unit].off = unit_base[unit], runs[unit].dbase = (int)list_base[unit];
 }
 
 // kglob[p] = kfirst[p] + max over segments of cnt(segment, p): the forward's search bound per ray.
@@ -1047,14 +1079,16 @@ __global__ void k_forward_bounds(Vol V, const int* kglob) {
 }
 
 // Target-major layout: segment t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
-// source deposits into it (the near block: all 125 bins); planes over the segment's x range.
-__global__ void k_target_totals(Vol V, long long* unit_tot) {
+// source deposits into it (the near block: all 125 bins); planes over the segment's x range, runs
+// without gaps as in the source-major copy.
+__global__ void k_target_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
     const int unit = blockIdx.x * blockDim.x + threadIdx.x;
     if (unit >= units) return;
     const int row = unit / V.nseg, seg = unit % V.nseg;
     const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
     long long off = 0;
+    int planes = 0;
     for (int p = 0; p < R1; ++p) {
         const int kt = V.rinfo[(size_t)unit * R1 + p].kt;
         const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : max(0, kt - gload(V.kfirst + p)));
@@ -1065,15 +1099,43 @@ __global__ void k_target_totals(Vol V, long long* unit_tot) {
         e->xa = x0;
         e->len = w;
         off += run_size(cnt, w);
+        planes += cnt;
     }
-    unit_tot[unit] = off;
+    unit_tot[unit] = align_up(off, kWave);
+    unit_planes[unit] = align_up(planes, kListAlign);
+    runs[unit].planes = planes;
 }
 
-__global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base) {
+__global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base,
+                                 const long long* __restrict__ list_base, PlaneRun* runs) {
     const int R1 = V.n_rays + 1;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)V.ny * V.nz * V.nseg * R1) return;
-    V.tinfo[id].off += unit_base[id / R1];
+    const long long unit = id / R1;
+    V.tinfo[id].off += unit_base[unit];
+    if (id % R1 == 0) runs[unit].off = unit_base[unit], runs[unit].dbase = (int)list_base[unit];
+}
+
+// Shift lists: the entry of step k of ray p's run is sign * D(p, kfirst(p) + k), segment by segment
+// in plane order; the alignment padding of each list is zero. One wave per segment.
+__global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __restrict__ info,
+                                                      const PlaneRun* __restrict__ runs, int sign,
+                                                      int* __restrict__ list) {
+    const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
+    const int unit = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
+    if (unit >= units) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const PlaneRun r = runs[unit];
+    if (r.planes == 0) return;
+    int* out = list + r.dbase;
+    int pos = 0;
+    for (int p = 0; p < R1; ++p) {
+        const int cnt = info[(size_t)unit * R1 + p].cnt;
+        const int* src = V.plin + (size_t)p * V.LP + gload(V.kfirst + p);
+        for (int k = lane; k < cnt; k += kWave) out[pos + k] = sign * src[k];
+        pos += cnt;
+    }
+    if (lane < align_up(pos, kListAlign) - pos) out[pos + lane] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1085,52 +1147,136 @@ __device__ inline float forward_scale(float f, float w) {
     return s > 0.0f ? s : 0.0f;
 }
 
-// Eight step offsets of one ray in a single scalar load (the offset table carries 8 entries of
-// slack past its end); entries past `count` are replaced by 0 so every gather stays in bounds.
-// Steps per group of the projector passes: their weight and gather loads are all issued
-// before the first is used.
-constexpr int kProjGroup = 16;
+// Plane-parallel streaming. A wave streams the plane sequence of one segment. Each lane loads 16
+// bytes of a plane: E = 16 / sizeof(WT) consecutive entries (x = c*E .. c*E + E - 1 with
+// c = lane % LPP), so one load instruction covers PPI = E planes, and a group of G loads covers
+// P = PPI * G = 32 planes. Lane sub-plane s = lane / LPP takes planes q + s*G + g (g < G) of the
+// group starting at plane q, so its G shifts are consecutive list entries (16-byte loads).
+// Each lane keeps E running sums; at the end the PPI lanes of each x chunk are added together.
+template <class WT>
+struct PlaneLanes {
+    static constexpr int E = 16 / (int)sizeof(WT);  // entries per lane and load
+    static constexpr int LPP = kWave / E;           // lanes per plane
+    static constexpr int PPI = E;                   // planes per load instruction
+    static constexpr int G = 32 / PPI;              // loads per group
+    static constexpr int P = PPI * G;               // planes per group
+    static_assert(G % kListAlign == 0, "a lane's shifts load as int4s from an aligned list");
+    static_assert(2 * P <= kListSlack, "the shift prefetch of a last group reads up to 2P - 1 past a list");
+};
 
-__device__ inline void load_offsets(const int* pl, int j, int count, int (&dl)[kProjGroup]) {
+// Byte offsets of the streaming loads are 32-bit per-lane values over a wave-uniform 64-bit base
+// (the global "saddr" form). Gathers address the vector through a base moved back by kGatherBias
+// bytes, so that the lane's offset (bias + (x + shift) * element size) is nonnegative for shifts
+// of either sign; shifts stay below 2^29 voxels in magnitude (a volume's density fits in 2^31
+// bytes), so the offset fits in 32 bits.
+constexpr unsigned kGatherBias = 0x80000000u;
+
+// 16 (or 8) bytes at any address aligned to the element type: the device reads them unaligned.
+typedef int v4i_u __attribute__((ext_vector_type(4), aligned(2)));
+typedef int v2i_u __attribute__((ext_vector_type(2), aligned(2)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+
+// N consecutive values of T (float or __half) at p, as floats.
+template <class T, int N>
+__device__ inline void load_values(const GLOBAL_AS char* p, float (&out)[N]) {
+    constexpr int bytes = N * (int)sizeof(T);
+    if constexpr (bytes % 16 == 0) {
 #pragma unroll
-    for (int h = 0; h < kProjGroup; h += 8) {
-        const v8i d = *(const GLOBAL_AS v8i*)(pl + j + h);
+        for (int h = 0; h < bytes / 16; ++h) {
+            const v4i_u r = *(const GLOBAL_AS v4i_u*)(p + 16 * h);
+            if constexpr (sizeof(T) == 2) {
+                const h8 x = __builtin_bit_cast(h8, r);
 #pragma unroll
-        for (int u = 0; u < 8; ++u) dl[h + u] = j + h + u < count ? d[u] : 0;
+                for (int e = 0; e < 8; ++e) out[8 * h + e] = (float)x[e];
+            } else {
+                const f4 x = __builtin_bit_cast(f4, r);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) out[4 * h + e] = x[e];
+            }
+        }
+    } else {
+        static_assert(bytes == 8, "loads of 8 or a multiple of 16 bytes");
+        const v2i_u r = *(const GLOBAL_AS v2i_u*)p;
+        if constexpr (sizeof(T) == 2) {
+            const h4 x = __builtin_bit_cast(h4, r);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) out[e] = (float)x[e];
+        } else {
+            const f2 x = __builtin_bit_cast(f2, r);
+            out[0] = x[0], out[1] = x[1];
+        }
     }
 }
 
-// Shared body of both operators: acc += sum over the stored steps of one ray of
-// w[step][lane] * vec[step shift + lane], where `wrow` is the ray's first plane at lane 0,
-// `len` the plane stride, and vec0 + sign * D(step) the uniform gather base.
+template <int G>
+__device__ inline void load_shifts(const GLOBAL_AS int* p, int (&d)[G]) {
+#pragma unroll
+    for (int h = 0; h < G; h += 4) {
+        const v4i x = *(const GLOBAL_AS v4i*)(p + h);
+        d[h] = x[0], d[h + 1] = x[1], d[h + 2] = x[2], d[h + 3] = x[3];
+    }
+}
+
+// acc[e] += sum over the segment's planes k of w[k][x] * vec0[x + shift(k)] for the lane's x =
+// c*E + e, all lanes ending with the full sums. `wseg` is the segment's first plane, `len` its
+// width (the plane stride), `list` its shifts. Zero weights (no source) are skipped by a select.
+// The last group clamps its plane reads to the last plane and zeroes the weights past it. Each
+// group loads the next group's shifts while its own loads are in flight, so shift reads run up to
+// 2P - 1 entries past the list (into the next list or the slack: valid shifts either way).
+// Lanes with x >= len read the next plane's entries; their sums are not used.
 template <class WT, class VT>
-__device__ inline float ray_planes(const WT* wrow, int len, int count, const int* pl, const VT* vec0, int sign,
-                                   int lane, float acc) {
-    for (int j = 0; j < count; j += kProjGroup) {
-        int dl[kProjGroup];
-        load_offsets(pl, j, count, dl);
-        float w[kProjGroup], v[kProjGroup];
+__device__ inline void segment_planes(const WT* wseg, int len, int planes, const int* list, const VT* vec0,
+                                      int lane, float (&acc)[PlaneLanes<WT>::E]) {
+    using L = PlaneLanes<WT>;
+    constexpr int E = L::E, LPP = L::LPP, G = L::G, P = L::P;
+    const int s = lane / LPP, c = lane % LPP;
 #pragma unroll
-        for (int u = 0; u < kProjGroup; ++u) {
-            const WT* wp = wrow + min(j + u, count - 1) * len;  // uniform plane base
-            const VT* vp = vec0 + sign * dl[u];                 // uniform gather base
-            w[u] = WeightIO<WT>::load(wp + lane);
-            v[u] = WeightIO<VT>::load(vp + lane);
+    for (int e = 0; e < E; ++e) acc[e] = 0.0f;
+    const GLOBAL_AS char* wq = (const GLOBAL_AS char*)wseg;
+    const size_t group_bytes = (size_t)P * len * sizeof(WT);
+    unsigned woff[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) woff[g] = (unsigned)(((s * G + g) * len + c * E) * (int)sizeof(WT));
+    const GLOBAL_AS char* vb = (const GLOBAL_AS char*)vec0 - kGatherBias;
+    const unsigned vlane = kGatherBias + (unsigned)(c * E * (int)sizeof(VT));
+    const GLOBAL_AS int* lp = (const GLOBAL_AS int*)list + s * G;
+    int d[G], dn[G];  // this lane's G shifts of the current and of the next group
+    load_shifts<G>(lp, d);
+    for (int q = 0; q < planes; q += P, wq += group_bytes) {
+        lp += P;
+        load_shifts<G>(lp, dn);
+        const bool full = q + P <= planes;
+        float w[G][E], v[G][E];
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            unsigned off = woff[g];
+            if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
+            load_values<WT, E>(wq + off, w[g]);
+            load_values<VT, E>(vb + (vlane + (unsigned)d[g] * (unsigned)sizeof(VT)), v[g]);
+        }
+        if (!full) {
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+                if (q + s * G + g >= planes)
+#pragma unroll
+                    for (int e = 0; e < E; ++e) w[g][e] = 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < kProjGroup; ++u) {
-            const bool m = (j + u < count) & (w[u] != 0.0f);
-            acc = m ? fmaf(w[u], v[u], acc) : acc;
-        }
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int e = 0; e < E; ++e) acc[e] = w[g][e] != 0.0f ? fmaf(w[g][e], v[g][e], acc[e]) : acc[e];
+#pragma unroll
+        for (int g = 0; g < G; ++g) d[g] = dn[g];
     }
-    return acc;
+#pragma unroll
+    for (int o = LPP; o < kWave; o <<= 1)
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[e] += __shfl_xor(acc[e], o);
 }
 
-// out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
-// streamed; for step (p, k) lane t gathers g at t - D(p,k), a wave-uniform shift. The input
-// vector carries a margin wide enough for every stored shift, so no gather needs a bounds
-// check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
-// and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 // Work distribution of the projector passes. The hardware places block b on XCD b % 8, and each
 // XCD has its own L2. The item list (rows in z-major order) is cut into one contiguous chunk per
 // XCD; a wave takes items from its own XCD's chunk, so the gathered neighbourhoods of the rows an
@@ -1161,30 +1307,27 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
 template <class WT, class VT>
 __device__ inline void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
                                     float* __restrict__ out, int lane) {
-    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
+    constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
     const int row = it.y, x0 = it.z;
-    float* o = out + V.vbase + (size_t)row * nx + x0;
+    float* o = out + V.vbase + (size_t)row * V.nx + x0;
     if (it.w < 0) {  // voxels outside the row's active range receive nothing (prc.cu:432 clears out)
         if (lane < -it.w) o[lane] = 0.0f;
         return;
     }
-    const RowInfo* tinfo = V.tinfo + ((size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave) * R1;
-    const int* plin = V.plin;
-    const int* kfirst = V.kfirst;
-    const WT* tb = (const WT*)V.tbuf;
-    const VT* g0 = gin + V.vbase + (size_t)row * nx + x0;
-    float acc = 0.0f;
-    for (int p = 0; p < R1; ++p) {
-        const RowInfo ti = gload(tinfo + p);
-        const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
-        acc = ray_planes(tb + ti.off, ti.len, ti.cnt, pl, g0, -1, lane, acc);
-    }
-    if (lane < it.w) o[lane] = acc;  // lanes past the segment read the next plane: not written
+    const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
+    const PlaneRun r = gload(V.truns + unit);
+    float acc[E];
+    segment_planes<WT, VT>((const WT*)V.tbuf + r.off, it.w, r.planes, V.tlist + r.dbase,
+                           gin + V.vbase + (size_t)row * V.nx + x0, lane, acc);
+    if (lane < LPP)
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (lane * E + e < it.w) o[lane * E + e] = acc[e];
 }
 
 // out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
-// streamed; for step (p, k) lane t gathers g at t - D(p,k), a wave-uniform shift. The input
-// vector carries a margin wide enough for every stored shift, so no gather needs a bounds
+// streamed; for step (p, k) target t gathers g at t - D(p,k), a shift shared by the plane. The
+// input vector carries a margin wide enough for every stored shift, so no gather needs a bounds
 // check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 // QUEUE selects the schedule: XCD-local work queues, or one wave per item in launch order.
@@ -1213,29 +1356,27 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
 template <class WT, class VT>
 __device__ inline void adjoint_item(const Vol& V, const int4 it, const VT* __restrict__ in,
                                     float* __restrict__ out, int lane) {
-    const int nx = V.nx, R1 = V.n_rays + 1, LP = V.LP;
+    constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
     const int row = it.y, x0 = it.z;
-    const RowInfo* rinfo = V.rinfo + ((size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave) * R1;
-    const int* plin = V.plin;
-    const int* kfirst = V.kfirst;
-    const WT* wb = (const WT*)V.wbuf;
-    const bool valid = lane < it.w;
-    const VT* s0 = in + V.vbase + (size_t)row * nx + x0;
-    float acc = 0.0f;
-    for (int p = 0; p < R1; ++p) {
-        const RowInfo ri = gload(rinfo + p);
-        const int* pl = plin + (size_t)p * LP + gload(kfirst + p);
-        acc = ray_planes(wb + ri.off, ri.len, ri.cnt, pl, s0, 1, lane, acc);
-    }
-    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
-    if (valid && gload(V.support + loc)) {
-        const float wv = gload(V.W + loc);
-        out[V.vbase + loc] = wv > 0.0f ? acc / wv : 0.0f;
-    }
+    const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
+    const PlaneRun r = gload(V.sruns + unit);
+    float acc[E];
+    segment_planes<WT, VT>((const WT*)V.wbuf + r.off, it.w, r.planes, V.slist + r.dbase,
+                           in + V.vbase + (size_t)row * V.nx + x0, lane, acc);
+    if (lane < LPP)
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const int x = lane * E + e;
+            const size_t loc = (size_t)row * V.nx + x0 + x;
+            if (x < it.w && gload(V.support + loc)) {
+                const float wv = gload(V.W + loc);
+                out[V.vbase + loc] = wv > 0.0f ? acc[e] / wv : 0.0f;
+            }
+        }
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
-// planes and gather in[] at s + D(p,k) (wave-uniform shift, margin-padded input).
+// planes and gather in[] at s + D(p,k) (one shift per plane, margin-padded input).
 template <class WT, class VT, bool QUEUE>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const VT* __restrict__ in,
@@ -1581,7 +1722,14 @@ struct prc_projector {
     int* kglob = nullptr;
     long long* row_tot = nullptr;
     long long* row_base = nullptr;
-    long long* total = nullptr;
+    long long* unit_planes = nullptr;  // per segment: list length (aligned), then its exclusive prefix
+    long long* list_base = nullptr;
+    long long* total = nullptr;  // entries (source, target), list lengths (source, target)
+    PlaneRun* sruns = nullptr;  // per segment: plane sequence of each copy
+    PlaneRun* truns = nullptr;
+    int* slist = nullptr;  // shift lists of the two copies (with kListSlack zero entries)
+    int* tlist = nullptr;
+    long long slcap = 0, tlcap = 0;  // list capacities (entries)
     int4* items_act = nullptr;  // active row segments (march, adjoint)
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
@@ -1592,7 +1740,7 @@ struct prc_projector {
     hipStream_t stream = nullptr;
     std::vector<hipStream_t> users;
     // Pinned staging of the sizes normalise_begin reads back (so the read-back does not block
-    // the host): total[2] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
+    // the host): total[4] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
     void* h_stage = nullptr;
     int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
     bool tuned = false;
@@ -1796,8 +1944,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LP = std::max(L, kNQ);
     A->fstep = dalloc<FarStep>((size_t)R * L);
     A->pstep = dalloc<int4>((size_t)R1 * LP);
-    A->plin = dalloc<int>((size_t)R1 * LP + kProjGroup);  // slack for the group-wide scalar loads
-    HIPCHK(hipMemset(A->plin, 0, ((size_t)R1 * LP + kProjGroup) * sizeof(int)));
+    A->plin = dalloc<int>((size_t)R1 * LP);
+    HIPCHK(hipMemset(A->plin, 0, (size_t)R1 * LP * sizeof(int)));
     A->kfirst = dalloc<int>(R1);
     A->kglob = dalloc<int>(R1);
     A->nstep = dalloc<NearStep>((size_t)kNearRays * kNearMax);
@@ -1829,9 +1977,14 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     A->tinfo = dalloc<RowInfo>(units * R1);
     A->row_tot = dalloc<long long>(units);
     A->row_base = dalloc<long long>(units);
-    A->total = dalloc<long long>(2);
+    A->unit_planes = dalloc<long long>(units);
+    A->list_base = dalloc<long long>(units);
+    A->sruns = dalloc<PlaneRun>(units);
+    A->truns = dalloc<PlaneRun>(units);
+    A->total = dalloc<long long>(4);
     V.rinfo = A->rinfo, V.tinfo = A->tinfo;
-    V.wbuf = nullptr, V.tbuf = nullptr;
+    V.sruns = A->sruns, V.truns = A->truns;
+    V.wbuf = nullptr, V.tbuf = nullptr, V.slist = nullptr, V.tlist = nullptr;
     A->dv = dalloc<Vol>(1);
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
 }
@@ -1848,6 +2001,18 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
     raw = (void*)dalloc<unsigned char>((size_t)(need + 2 * kWave) * wbytes);
     buf = (unsigned char*)raw + (size_t)kWave * wbytes;
     cap = need;
+}
+
+// Shift lists, with kListSlack entries of slack past the last list. Allocated zeroed; entries past
+// the current lists hold zeros or shifts of an earlier normalisation, valid shifts either way (the
+// gather margin never changes after the first normalisation).
+void ensure_list(int*& list, long long& cap, long long entries) {
+    if (list && entries <= cap) return;
+    if (list) (void)hipFree(list);
+    list = nullptr, cap = 0;
+    list = dalloc<int>((size_t)(entries + kListSlack));
+    HIPCHK(hipMemset(list, 0, (size_t)(entries + kListSlack) * sizeof(int)));
+    cap = entries;
 }
 
 // The normalisation pass (Projector::Projector, prc.cu:405-422): count, lay out both copies of
@@ -1880,20 +2045,23 @@ void normalise_begin(prc_projector* A) {
     else launch_far<float>(A, 1);
     {
         ProfScope ps("layout", st);
-        k_row_totals<<<blocks_for((long long)units, 256), 256, 0, st>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024, 0, st>>>(A->row_tot, A->row_base, (int)units, A->total);
-        k_row_offsets<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->row_base);
+        const unsigned nu = blocks_for((long long)units, 256), nr = blocks_for((long long)units * R1, 256);
+        k_row_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns);
+        k_scan_units<<<1, 1024, 0, st>>>(A->row_tot, A->unit_planes, A->row_base, A->list_base, (int)units,
+                                         A->total, A->total + 2);
+        k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
         k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
-        k_forward_bounds<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->kglob);
-        k_target_totals<<<blocks_for((long long)units, 256), 256, 0, st>>>(V, A->row_tot);
-        k_scan_rows<<<1, 1024, 0, st>>>(A->row_tot, A->row_base, (int)units, A->total + 1);
-        k_target_offsets<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(V, A->row_base);
+        k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob);
+        k_target_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
+        k_scan_units<<<1, 1024, 0, st>>>(A->row_tot, A->unit_planes, A->row_base, A->list_base, (int)units,
+                                         A->total + 1, A->total + 3);
+        k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns);
         HIPCHK(hipGetLastError());
     }
-    if (!A->h_stage) HIPCHK(hipHostMalloc(&A->h_stage, 2 * sizeof(long long) + (2 + R1) * sizeof(int)));
+    if (!A->h_stage) HIPCHK(hipHostMalloc(&A->h_stage, 4 * sizeof(long long) + (2 + R1) * sizeof(int)));
     long long* total = (long long*)A->h_stage;
-    int* ctr = (int*)(total + 2);
-    HIPCHK(hipMemcpyAsync(total, A->total, 2 * sizeof(long long), hipMemcpyDeviceToHost, st));
+    int* ctr = (int*)(total + 4);
+    HIPCHK(hipMemcpyAsync(total, A->total, 4 * sizeof(long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(ctr + 2, A->kglob, R1 * sizeof(int), hipMemcpyDeviceToHost, st));
     if (A->cls) HIPCHK(hipMemcpyAsync(ctr, A->memo_meta + 2 * kMaxTemplates, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
 }
@@ -1905,7 +2073,7 @@ void normalise_finish(prc_projector* A) {
     hipStream_t st = A->stream;
     const int R1 = V.n_rays + 1;
     const long long* total = (const long long*)A->h_stage;
-    const int* ctr = (const int*)(total + 2);
+    const int* ctr = (const int*)(total + 4);
     const std::vector<int> kglob(ctr + 2, ctr + 2 + R1);
     if (A->cls) {
         A->n_list = ctr[0];
@@ -1921,12 +2089,23 @@ void normalise_finish(prc_projector* A) {
     else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
     ensure_weights(A->wraw, A->wbuf, A->wcap, total[0], A->wbytes);
     ensure_weights(A->traw, A->tbuf, A->tcap, total[1], A->wbytes);
+    if (total[2] > INT_MAX - kListSlack || total[3] > INT_MAX - kListSlack)
+        fail(PRC_ERR_INVALID, "the system matrix holds too many planes for 32-bit shift-list offsets");
+    ensure_list(A->slist, A->slcap, total[2]);
+    ensure_list(A->tlist, A->tlcap, total[3]);
     A->stats.weight_entries = total[0];
     A->stats.target_entries = total[1];
     A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
-    V.wbuf = A->wbuf, V.tbuf = A->tbuf;
+    V.wbuf = A->wbuf, V.tbuf = A->tbuf, V.slist = A->slist, V.tlist = A->tlist;
     HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
+    {
+        const size_t units = (size_t)V.ny * V.nz * V.nseg;
+        const unsigned nb = blocks_for((long long)units, 256 / kWave);
+        k_plane_lists<<<nb, 256, 0, st>>>(V, A->rinfo, A->sruns, 1, A->slist);
+        k_plane_lists<<<nb, 256, 0, st>>>(V, A->tinfo, A->truns, -1, A->tlist);
+        HIPCHK(hipGetLastError());
+    }
     if (A->wbytes == 2) {
         launch_near<__half>(A);
         launch_far<__half>(A, 2);
@@ -1948,65 +2127,84 @@ void normalise_projectors(prc_projector* const* As, int n) {
 void normalise_projector(prc_projector* A) { normalise_projectors(&A, 1); }
 
 // Host replay of the layout and of every index range the forward and adjoint read (PRC_AUDIT=1):
-// each (segment, ray) run must follow the previous one in the buffer, plane reads must stay
-// inside the buffer's one-wave slack, offset-table reads inside its spare entries, and every
-// gather shift inside the vectors' margin. Used by the tests before any operator launch; fails
+// the runs of each segment must follow one another from its 64-entry-aligned start, its plane
+// sequence and shift list must match the runs (every shift equal to +-D of its ray and step, and
+// inside the vectors' margin), and plane reads must stay inside the buffer's one-wave slack and
+// shift reads inside the list's slack. Used by the tests before any operator launch; fails
 // instead of launching a kernel that could read out of bounds.
 void audit_layout(prc_projector* A) {
     const Vol& V = A->hv;
     const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1, LP = V.LP;
     std::vector<RowInfo> ri(units * R1), ti(units * R1);
-    std::vector<int> kf(R1), pl((size_t)R1 * LP + kProjGroup);
+    std::vector<int> kf(R1), pl((size_t)R1 * LP);
     HIPCHK(hipMemcpy(ri.data(), A->rinfo, ri.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(ti.data(), A->tinfo, ti.size() * sizeof(RowInfo), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(kf.data(), A->kfirst, R1 * sizeof(int), hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(pl.data(), A->plin, pl.size() * sizeof(int), hipMemcpyDeviceToHost));
-    auto check_layout = [&](const std::vector<RowInfo>& info, long long total, const char* what) {
-        long long next = 0;
+    const long long shift_max = A->margin - kWave;
+    const int P = std::max(PlaneLanes<float>::P, PlaneLanes<__half>::P);
+    auto check_copy = [&](const std::vector<RowInfo>& info, const PlaneRun* druns, const int* dlist, long long total,
+                          long long list_len, int sign, const char* what) {
+        std::vector<PlaneRun> runs(units);
+        std::vector<int> list((size_t)list_len);
+        HIPCHK(hipMemcpy(runs.data(), druns, units * sizeof(PlaneRun), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(list.data(), dlist, list.size() * sizeof(int), hipMemcpyDeviceToHost));
+        long long next = 0, lnext = 0;
         for (size_t u = 0; u < units; ++u) {
             const int row = (int)(u / V.nseg), seg = (int)(u % V.nseg);
             const int w = std::min(kWave, std::max(0, A->h_row_len[row] - kWave * seg));
+            const PlaneRun& r = runs[u];
+            if (r.off != next || r.dbase != lnext)
+                fail(PRC_ERR_INVALID, "audit: %s plane sequence of segment %zu is misplaced", what, u);
+            long long off = next;
+            int planes = 0;
             for (int p = 0; p < R1; ++p) {
                 const RowInfo& e = info[u * R1 + p];
-                if (e.off != next || e.len != w || (w > 0 && e.xa != A->h_row_xa[row] + kWave * seg) || e.cnt < 0)
+                if (e.off != off || e.len != w || (w > 0 && e.xa != A->h_row_xa[row] + kWave * seg) || e.cnt < 0 ||
+                    (w == 0 && e.cnt != 0) || (p > 0 && kf[p] + e.cnt > LP))
                     fail(PRC_ERR_INVALID, "audit: %s layout of segment %zu ray %d is inconsistent", what, u, p);
-                next += run_size(e.cnt, e.len);
+                for (int k = 0; k < e.cnt; ++k) {
+                    const long long d = pl[(size_t)p * LP + kf[p] + k];
+                    if (d > shift_max || d < -shift_max)
+                        fail(PRC_ERR_INVALID, "audit: %s shift %lld of ray %d exceeds the margin", what, d, p);
+                    if (list[(size_t)(lnext + planes + k)] != sign * d)
+                        fail(PRC_ERR_INVALID, "audit: %s shift list of segment %zu ray %d step %d is wrong", what, u, p, k);
+                }
+                off += run_size(e.cnt, e.len);
+                planes += e.cnt;
             }
+            if (r.planes != planes) fail(PRC_ERR_INVALID, "audit: %s segment %zu counts %d planes, not %d", what, u, r.planes, planes);
+            for (long long k = planes; k < align_up(planes, kListAlign); ++k)
+                if (list[(size_t)(lnext + k)] != 0) fail(PRC_ERR_INVALID, "audit: %s list padding is not zero", what);
+            // Plane reads: lanes read up to one wave past the last plane's start; shift reads (the
+            // last group's prefetch) up to 2P - 2 entries past the list's end.
+            if (planes > 0 && (r.off < 0 || r.off + (long long)(planes - 1) * w + kWave - 1 >= total + kWave))
+                fail(PRC_ERR_INVALID, "audit: %s planes of segment %zu read outside the buffer", what, u);
+            if (lnext + planes + 2 * P - 2 >= list_len + kListSlack)
+                fail(PRC_ERR_INVALID, "audit: %s shifts of segment %zu read outside the list", what, u);
+            next = align_up(off, kWave);
+            lnext += align_up(planes, kListAlign);
         }
-        if (next != total) fail(PRC_ERR_INVALID, "audit: %s layout holds %lld entries, not %lld", what, next, total);
+        if (next != total || lnext != list_len)
+            fail(PRC_ERR_INVALID, "audit: %s layout holds %lld entries and %lld shifts, not %lld and %lld", what, next,
+                 lnext, total, list_len);
     };
-    check_layout(ri, A->stats.weight_entries, "source-major");
-    check_layout(ti, A->stats.target_entries, "target-major");
-    const long long shift_max = A->margin - kWave;
-    auto check_item = [&](const std::vector<RowInfo>& info, long long total, const int4& it, const char* what) {
+    const long long* totals = (const long long*)A->h_stage;  // entries and list lengths of this layout
+    check_copy(ri, A->sruns, A->slist, A->stats.weight_entries, totals[2], 1, "source-major");
+    check_copy(ti, A->truns, A->tlist, A->stats.target_entries, totals[3], -1, "target-major");
+    auto check_item = [&](const std::vector<RowInfo>& info, const int4& it, const char* what) {
         const int row = it.y, x0 = it.z;
         const int seg = (x0 - A->h_row_xa[row]) / kWave;
         if (A->h_row_len[row] <= 0 || (x0 - A->h_row_xa[row]) % kWave || seg < 0 || seg >= V.nseg)
             fail(PRC_ERR_INVALID, "audit: %s item at row %d x %d is not a segment", what, row, x0);
-        for (int p = 0; p < R1; ++p) {
-            const RowInfo& e = info[((size_t)row * V.nseg + seg) * R1 + p];
-            if (e.xa != x0 || e.len != it.w) fail(PRC_ERR_INVALID, "audit: %s item and segment disagree", what);
-            if (e.cnt <= 0) continue;
-            const long long lo = e.off;
-            const long long hi = e.off + (long long)(e.cnt - 1) * e.len + kWave - 1;
-            if (lo < 0 || hi >= total + kWave)
-                fail(PRC_ERR_INVALID, "audit: %s planes of row %d ray %d read [%lld, %lld] outside [0, %lld)", what,
-                     row, p, lo, hi, total + kWave);
-            const long long last = (long long)p * LP + kf[p] + ((e.cnt - 1) / kProjGroup) * kProjGroup + kProjGroup - 1;
-            if (kf[p] + e.cnt > LP || last >= (long long)pl.size())
-                fail(PRC_ERR_INVALID, "audit: %s offsets of ray %d read past the table", what, p);
-            for (int j = 0; j < e.cnt; ++j) {
-                const long long d = pl[(size_t)p * LP + kf[p] + j];
-                if (d > shift_max || d < -shift_max)
-                    fail(PRC_ERR_INVALID, "audit: %s shift %lld of ray %d exceeds the margin", what, d, p);
-            }
-        }
+        const RowInfo& e = info[((size_t)row * V.nseg + seg) * R1];
+        if (e.xa != x0 || e.len != it.w) fail(PRC_ERR_INVALID, "audit: %s item and segment disagree", what);
     };
     for (const int4& it : A->h_items_fwd)
-        if (it.w > 0) check_item(ti, A->stats.target_entries, it, "forward");
+        if (it.w > 0) check_item(ti, it, "forward");
     for (const int4& it : A->h_items_act)
-        if (it.w > 0) check_item(ri, A->stats.weight_entries, it, "adjoint");
+        if (it.w > 0) check_item(ri, it, "adjoint");
 }
 
 void read_counters(prc_projector* A) {
@@ -2024,6 +2222,7 @@ void release(prc_projector* A) {
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
+    dfree(A->unit_planes), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
     if (A->traw) (void)hipFree(A->traw);
