@@ -48,6 +48,8 @@ constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
 constexpr int kWave = 64;
 constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
 constexpr int kFarGroup = 8;             // far-march steps whose density loads are issued together
+constexpr int kCountGroup = 16;          // the same for the count pass
+constexpr int kMarchGroup = kCountGroup > kFarGroup ? kCountGroup : kFarGroup;
 
 // ---------------------------------------------------------------------------------------
 // Isotope model (prc.cu:38-57): Paneque-Yunta et al., Phys. Med. Biol. 70 (2025) 165013.
@@ -171,6 +173,7 @@ struct Vol {            // device descriptor of one volume
     int nx, ny, nz, n_rays;
     float sx, sy, sz, cutoff, support_rho, air_cap, scale;
     int nb, exact, L, LP;
+    int LF;             // step-table stride: L rounded up to kMarchGroup, zero steps past L
     Branch br[3];
     size_t vbase;       // offset of this volume in concatenated voxel vectors
     const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, +inf halo
@@ -190,7 +193,7 @@ struct Vol {            // device descriptor of one volume
     const PlaneRun* truns;  // [ny*nz*nseg] target-major plane sequence of each segment
     const int* slist;       // source-major shifts +D (the adjoint gathers at s + D)
     const int* tlist;       // target-major shifts -D (the forward gathers at t - D)
-    const FarStep* fstep;   // [n_rays][L]
+    const FarStep* fstep;   // [n_rays][LF]
     const int4* pstep;      // [n_rays + 1][LP] (dx, dy, dz) of each step; p = 0 is the near block
     const int* plin;        // [n_rays + 1][LP] linear voxel offset (dz*ny + dy)*nx + dx of each step
     const int* kfirst;      // [n_rays + 1] first stored step
@@ -211,6 +214,7 @@ struct Scal {
 // pointers to the compiler; casting them lets wave-uniform reads become scalar loads and lets
 // per-lane accesses use global (not flat) instructions.
 #define GLOBAL_AS __attribute__((address_space(1)))
+#define CONST_AS __attribute__((address_space(4)))  // global memory that no kernel writes: scalar loads
 typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
@@ -361,7 +365,7 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirs
             pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
             plin[(size_t)(1 + id) * V.LP + s] = linear_offset(V, i, j, k);
             const float seg = w.step();
-            fstep[(size_t)id * V.L + s] = FarStep{seg, (k * nyp + j) * nxp + i, (short)i, (short)j, (short)k, 0};
+            fstep[(size_t)id * V.LF + s] = FarStep{seg, (k * nyp + j) * nxp + i, (short)i, (short)j, (short)k, 0};
         }
         kfirst[1 + id] = kf < 0 ? V.L : kf;
     } else if (id < V.n_rays + kNearRays) {
@@ -807,14 +811,80 @@ __device__ inline long long lane_of(long long v, int l) {
     return (long long)(((unsigned long long)(unsigned)hi << 32) | lo);
 }
 
-// Far rays. PASS 1 counts the stored steps of every (row, ray); PASS 2 writes the weights and
-// finishes W (continuing from the near sum, in the reference's accumulation order). The step
-// table is read 64 steps at a time, one step per lane, and each step's values are broadcast
-// from its lane; in PASS 2 the same lane also resolves the step's target row (the target-major
-// plane offset), so a step costs one density load and the two weight stores.
-template <int PASS, bool EXACT, class WT>
-__global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const int4* __restrict__ items,
-                                              int n_items, unsigned long long* vrs, unsigned long long* ndep) {
+// Far rays, count pass: the number of stored steps of every (segment, ray), i.e. one past the last
+// step at which some source of the segment deposits, less kfirst. The count only sizes the layout:
+// an over-count stores extra zero planes and changes no result. A step deposits (prc.cu:146-163)
+// only if the lane is alive (range below the cutoff), inside the grid, in tissue and past kfirst;
+// the count takes the last such step, an upper bound of the last deposit that needs neither the
+// survival table nor its lookup. The march itself (range, air length, death) is the fill pass's.
+// One wave per segment; step lengths and shifts come from scalar loads (the step table is read
+// through the constant address space: it is not written while kernels run). COUNT adds the
+// number of voxel visits of live lanes to *vrs.
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                    int n_items, const FarStep* __restrict__ fstep,
+                                                    const int* __restrict__ kfirst, RowInfo* __restrict__ rinfo,
+                                                    unsigned long long* vrs) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    if (it.w <= 0) return;
+    const Vol& V = vols[it.x];
+    const int ny = V.ny, n_rays = V.n_rays, R1 = n_rays + 1, LF = V.LF;
+    const float cutoff = V.cutoff, support_rho = V.support_rho, air_cap = V.air_cap;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const bool act = valid && gload(V.support + (size_t)row * V.nx + x0 + lane);
+    // Density reads through a buffer descriptor over the padded allocation: every lane loads every
+    // step of a group, in bounds by the allocation's margin (a group runs only while a lane is alive).
+    const __amdgpu_buffer_rsrc_t rho_rsrc = buffer_of(V.rho_base, V.rho_bytes);
+    const int rho_idx = (int)(V.rho_p - V.rho_base) +
+                        (int)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
+    const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
+    unsigned visits = 0;
+    for (int r = 0; r < n_rays; ++r) {
+        const int kf = kfirst[1 + r];
+        const FarStep* fs = fstep + (size_t)r * LF;
+        float w = act ? 0.0f : cutoff, air = 0.0f;
+        int last = -1;
+        for (int s0 = 0; s0 < LF && __any(w < cutoff); s0 += kCountGroup) {
+            float seg[kCountGroup], rv[kCountGroup];
+#pragma unroll
+            for (int u = 0; u < kCountGroup; ++u) {
+                const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, dp: a scalar load
+                seg[u] = __builtin_bit_cast(float, f[0]);
+                rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + f[1]) * 4);
+            }
+#pragma unroll
+            for (int u = 0; u < kCountGroup; ++u) {
+                const float rho = rv[u];
+                const bool inside = rho < INFINITY, tissue = rho >= support_rho, alive = w < cutoff;
+                if (COUNT) visits += (inside && alive) ? 1u : 0u;
+                last = (inside && tissue && alive && s0 + u >= kf) ? s0 + u : last;
+                air = tissue ? 0.0f : air + seg[u];
+                const bool dies = !inside || (!tissue && air > air_cap);
+                w = dies ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
+            }
+        }
+        // One wave per segment: its count is the segment's.
+        const int kmax = wave_max(last + 1);
+        if (lane == 0 && kmax > kf) rinfo[unit * R1 + 1 + r].cnt = kmax - kf;
+    }
+    if (COUNT && vrs) {
+        const long long tot = wave_sum_ll((long long)visits);
+        if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
+    }
+}
+
+// Far rays, fill pass: writes the weights and finishes W (continuing from the near sum, in the
+// reference's accumulation order). The step table is read 64 steps at a time, one step per lane,
+// and each step's values are broadcast from its lane; the same lane also resolves the step's
+// target row (the target-major plane offset), so a step costs one density load and the two weight
+// stores. COUNT adds the number of stored positive far weights to *ndep.
+template <bool EXACT, class WT, bool COUNT>
+__global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                   int n_items, unsigned long long* ndep) {
     __shared__ float tab[kTabLds];
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     const int4 it = items[min(item_id, n_items - 1)];
@@ -822,7 +892,7 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     if (!EXACT) load_table(tab, V);
     if (item_id >= n_items || it.w <= 0) return;
     const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
-    const int ny = V.ny, nz = V.nz, L = V.L, n_rays = V.n_rays, R1 = n_rays + 1;
+    const int ny = V.ny, nz = V.nz, L = V.L, LF = V.LF, n_rays = V.n_rays, R1 = n_rays + 1;
     const FarStep* fstep = V.fstep;
     const int* kfirst = V.kfirst;
     const RowInfo* rinfo = V.rinfo;
@@ -834,8 +904,8 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     const bool valid = lane < it.w;
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
-    // Density reads go through a buffer descriptor over the whole padded allocation: lane index
-    // plus the step's offset, as one 32-bit add (every index stays inside the allocation).
+    // Density reads through a buffer descriptor over the padded allocation: every lane loads every
+    // step of a group, in bounds by the allocation's margin (a group runs only while a lane is alive).
     const __amdgpu_buffer_rsrc_t rho_rsrc = buffer_of(V.rho_base, V.rho_bytes);
     const int rho_idx = (int)(V.rho_p - V.rho_base) +
                         (int)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
@@ -844,32 +914,27 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
     const int* row_xa = V.row_xa;
     const int* row_len = V.row_len;
     unsigned visits = 0, deposits = 0;
-    float wacc = (PASS == 2 && act) ? gload(V.W + loc) : 0.0f;
+    float wacc = act ? gload(V.W + loc) : 0.0f;
 
     for (int r = 0; r < n_rays; ++r) {
         const int kf = gload(kfirst + 1 + r);
-        const FarStep* fs = fstep + (size_t)r * L;
-        int cnt = 0;
-        WT* wp = nullptr;
-        if (PASS == 2) {
-            const RowInfo ri = gload(rinfo + unit * R1 + 1 + r);
-            cnt = ri.cnt;
-            if (cnt <= 0) continue;  // no source of this segment deposits along this ray
-            wp = wbuf + ri.off - (long long)kf * ri.len + lane;
-        }
-        const int len = PASS == 2 ? it.w : 0;  // plane width = the segment's width
-        // PASS 2 marches only up to the row's last stored step.
-        const int send = PASS == 2 ? kf + cnt : L;
+        const FarStep* fs = fstep + (size_t)r * LF;
+        const RowInfo ri = gload(rinfo + unit * R1 + 1 + r);
+        const int cnt = ri.cnt;
+        if (cnt <= 0) continue;  // no source of this segment deposits along this ray
+        WT* wp = wbuf + ri.off - (long long)kf * ri.len + lane;
+        const int len = it.w;  // plane width = the segment's width
+        // March only up to the row's last stored step.
+        const int send = kf + cnt;
         Lane st = M.start(act);
-        int last = -1;
         int s = 0;
         bool going = __any(act);
         for (int s0 = 0; s0 < send && going; s0 += kWave) {
             const int sl = s0 + lane;
             const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, 0, 0, 0, 0, 0};
-            // PASS 2: target-major slots of step sl (its target row's one or two segments).
+            // Target-major slots of step sl (its target row's one or two segments).
             TargetPair tl{0, 0, kWave, 0};
-            if (PASS == 2 && sl >= kf && sl < send) {
+            if (sl >= kf && sl < send) {
                 const int ty = y + fl.dy, tz = z + fl.dz;
                 if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)nz) {
                     const int t = tz * ny + ty;
@@ -879,50 +944,41 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
             }
             const int nst = min(kWave, send - s0);
             for (int j = 0; j < nst; j += kFarGroup) {
-                // Density loads of the group first. A lane alive at the group start is inside the
-                // grid, so at most kFarGroup voxels outside it by the group's end: inside the +inf
-                // margin of the padded density.
-                // Every lane loads (a dead lane its own voxel, whose density it ignores), so the
-                // loads issue back to back.
-                const int go = M.alive(st) ? -1 : 0;  // dead lanes read their own voxel
                 float seg[kFarGroup], rv[kFarGroup];
 #pragma unroll
                 for (int u = 0; u < kFarGroup; ++u) {
                     seg[u] = lane_of(fl.seg, j + u);
-                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + (lane_of(fl.dp, j + u) & go)) * 4);
+                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + lane_of(fl.dp, j + u)) * 4);
                 }
                 // Steps past nst are marched too, without stores: past the row's last stored step
-                // no lane of the row deposits (that is what the count pass measured), and past the
+                // no lane of the row deposits (that is what the count pass bounded), and past the
                 // table's end every lane is dead (a ray leaves the grid within nx+ny+nz steps).
                 WT hv[kFarGroup];
 #pragma unroll
                 for (int u = 0; u < kFarGroup; ++u) {
                     const int su = s0 + j + u;
                     const float dep = march_step<EXACT>(st, rv[u], seg[u], su >= kf, M, tab, visits);
-                    if (PASS == 1) {
-                        if (dep > 0.0f) last = su;
-                    } else {
-                        hv[u] = WeightIO<WT>::store_value(dep);
-                        const float wv = WeightIO<WT>::to_float(hv[u]);
-                        if (wv > 0.0f) wacc += wv, ++deposits;
+                    hv[u] = WeightIO<WT>::store_value(dep);
+                    const float wv = WeightIO<WT>::to_float(hv[u]);
+                    if (wv > 0.0f) {
+                        wacc += wv;
+                        if (COUNT) ++deposits;
                     }
                 }
                 // The group's stores go out after its last density wait, so no wait of this
                 // group waits on a store.
-                if (PASS == 2) {
 #pragma unroll
-                    for (int u = 0; u < kFarGroup; ++u) {
-                        const int su = s0 + j + u;
-                        if (j + u >= nst) break;
-                        if (su < kf) continue;
-                        if (valid) gstore(wp + (size_t)su * len, hv[u]);
-                        // Target-major copy: step su of this source is step su of its target. A
-                        // deposit implies an active target inside its row's active range.
-                        const TargetPair tp{lane_of(tl.b0, j + u), lane_of(tl.b1, j + u), lane_of(tl.thr, j + u),
-                                            lane_of(tl.ok, j + u)};
-                        long long idx;
-                        if (target_slot(tp, lane, idx) && WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + idx, hv[u]);
-                    }
+                for (int u = 0; u < kFarGroup; ++u) {
+                    const int su = s0 + j + u;
+                    if (j + u >= nst) break;
+                    if (su < kf) continue;
+                    if (valid) gstore(wp + (size_t)su * len, hv[u]);
+                    // Target-major copy: step su of this source is step su of its target. A
+                    // deposit implies an active target inside its row's active range.
+                    const TargetPair tp{lane_of(tl.b0, j + u), lane_of(tl.b1, j + u), lane_of(tl.thr, j + u),
+                                        lane_of(tl.ok, j + u)};
+                    long long idx;
+                    if (target_slot(tp, lane, idx) && WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + idx, hv[u]);
                 }
                 s = s0 + j + kFarGroup;
                 if (!__any(M.alive(st))) {
@@ -931,21 +987,11 @@ __global__ __launch_bounds__(256) void k_far(const Vol* __restrict__ vols, const
                 }
             }
         }
-        if (PASS == 1) {
-            // One wave per segment: its count is the segment's.
-            const int kmax = wave_max(last + 1);
-            if (lane == 0 && kmax > kf) V.rinfo[unit * R1 + 1 + r].cnt = kmax - kf;
-        } else {
-            for (s = max(s, kf); s < send; ++s)
-                if (valid) gstore(wp + (size_t)s * len, WeightIO<WT>::store_value(0.0f));
-        }
+        for (s = max(s, kf); s < send; ++s)
+            if (valid) gstore(wp + (size_t)s * len, WeightIO<WT>::store_value(0.0f));
     }
-    if (PASS == 2 && act) gstore(V.W + loc, wacc);
-    if (PASS == 1 && vrs) {
-        const long long tot = wave_sum_ll((long long)visits);
-        if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
-    }
-    if (PASS == 2 && ndep) {
+    if (act) gstore(V.W + loc, wacc);
+    if (COUNT && ndep) {
         const long long tot = wave_sum_ll((long long)deposits);
         if (lane == 0 && tot) atomicAdd(ndep, (unsigned long long)tot);
     }
@@ -1743,6 +1789,7 @@ struct prc_projector {
     // the host): total[4] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
     void* h_stage = nullptr;
     int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
+    bool counted = false;  // march statistics read (after the first normalisation)
     bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
     std::vector<int4> h_items_act, h_items_all, h_items_fwd;
@@ -1840,16 +1887,24 @@ void build_items(prc_projector* A) {
 
 template <class WT>
 void launch_far(prc_projector* A, int pass) {
-    const bool exact = A->hv.exact;
+    const bool exact = A->hv.exact, count = !A->counted;
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
+    hipStream_t st = A->stream;
     if (pass == 1) {
-        ProfScope ps("far_count", A->stream);
-        if (exact) k_far<1, true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
-        else k_far<1, false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters + 1, nullptr);
+        ProfScope ps("far_count", st);
+        if (count)
+            k_far_count<true><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, A->fstep, A->kfirst, A->rinfo,
+                                                  A->counters + 1);
+        else
+            k_far_count<false><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, A->fstep, A->kfirst, A->rinfo,
+                                                   nullptr);
     } else {
-        ProfScope ps("far_fill", A->stream);
-        if (exact) k_far<2, true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
-        else k_far<2, false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, nullptr, A->counters + 2);
+        ProfScope ps("far_fill", st);
+        unsigned long long* ndep = A->counters + 2;
+        if (exact && count) k_far_fill<true, WT, true><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, ndep);
+        else if (exact) k_far_fill<true, WT, false><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, nullptr);
+        else if (count) k_far_fill<false, WT, true><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, ndep);
+        else k_far_fill<false, WT, false><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, nullptr);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1890,8 +1945,12 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     // Density: clamp (prc.cu:603), support mask (prc.cu:605), +inf halo of width one voxel.
     const int nxp = g.nx + 2, nyp = g.ny + 2, nzp = g.nz + 2;
     const size_t np = (size_t)nxp * nyp * nzp;
-    // +inf margin of (kFarGroup + 1) padded slices on both sides for the march's prefetches.
-    const size_t margin = (size_t)(kFarGroup + 1) * nxp * nyp;
+    // +inf margin on both sides for the march's group loads: every lane of a wave loads the
+    // group's voxels, and a group runs only while some lane of the wave is alive. That lane's
+    // last voxel is inside the grid and a group moves it at most kMarchGroup voxels, at most
+    // kMarchGroup padded slices; the other lanes of the wave are at most 63 voxels further along
+    // the row. So (kMarchGroup + 1) slices and two waves of margin hold every group load.
+    const size_t margin = (size_t)(kMarchGroup + 1) * nxp * nyp + 2 * kWave;
     if ((np + 2 * margin) * sizeof(float) > (size_t)INT_MAX)
         fail(PRC_ERR_INVALID, "Volume too large for 32-bit density addressing");
     std::vector<float> rp(np + 2 * margin, INFINITY);
@@ -1942,7 +2001,9 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int R = g.n_rays, R1 = R + 1;
     const int L = g.nx + g.ny + g.nz;  // a ray leaves any grid within nx+ny+nz voxel visits
     const int LP = std::max(L, kNQ);
-    A->fstep = dalloc<FarStep>((size_t)R * L);
+    const int LF = (int)align_up(L, kMarchGroup);
+    A->fstep = dalloc<FarStep>((size_t)R * LF);
+    HIPCHK(hipMemset(A->fstep, 0, (size_t)R * LF * sizeof(FarStep)));  // steps past L: no move, no length
     A->pstep = dalloc<int4>((size_t)R1 * LP);
     A->plin = dalloc<int>((size_t)R1 * LP);
     HIPCHK(hipMemset(A->plin, 0, (size_t)R1 * LP * sizeof(int)));
@@ -1959,7 +2020,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     V.nb = A->model.n;
     for (int i = 0; i < 3; ++i) V.br[i] = i < A->model.n ? A->model.br[i] : Branch{0, 0, 0, 0, 0};
     V.exact = (g.flags & PRC_FLAG_EXACT_RANGE) != 0;
-    V.L = L, V.LP = LP, V.vbase = 0;
+    V.L = L, V.LP = LP, V.LF = LF, V.vbase = 0;
     V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
     V.rho_base = A->rho_alloc, V.rho_bytes = (long long)(np + 2 * margin) * (long long)sizeof(float);
     V.fstep = A->fstep, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
@@ -2117,11 +2178,17 @@ void normalise_finish(prc_projector* A) {
 
 // The normalisation pass (Projector::Projector, prc.cu:405-422) of several projectors at once:
 // each runs on its own stream, so their kernels fill each other's tails. Returns when all are done.
+void read_counters(prc_projector* A);
+
 void normalise_projectors(prc_projector* const* As, int n) {
     for (int i = 0; i < n; ++i) normalise_begin(As[i]);
     for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
     for (int i = 0; i < n; ++i) normalise_finish(As[i]);
     for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
+    // The march statistics depend only on the density and the geometry, which a projector keeps:
+    // they are counted by its first normalisation and kept.
+    for (int i = 0; i < n; ++i)
+        if (!As[i]->counted) read_counters(As[i]), As[i]->counted = true;
 }
 
 void normalise_projector(prc_projector* A) { normalise_projectors(&A, 1); }
@@ -2690,7 +2757,6 @@ int prc_projector_table(prc_projector* A, float* host_table, float* scale) {
 int prc_projector_stats(prc_projector* A, prc_stats* stats) {
     return guarded([&] {
         if (!A || !stats) fail(PRC_ERR_INVALID, "null argument");
-        read_counters(A);
         *stats = A->stats;
     });
 }
