@@ -482,23 +482,22 @@ __device__ inline bool target_slot(const TargetPair& tp, int lane, long long& id
 }
 
 // The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into the shell
-// registers (shell 4 split in two halves of 18 to stay within register indexing). rho_at(dp)
-// returns the density at padded offset dp from the lane's source.
-#define NEAR_ACCUMULATORS float a0 = 0.0f, a1[6], a2[18], a3[32], a4l[18], a4h[18], a5[24], a6[8]
-#define NEAR_ARGS a0, a1, a2, a3, a4l, a4h, a5, a6
+// registers: arrays of 18 to 32 entries (shells 1-2 and 5-6 share one, shell 4 is split in two
+// halves of 18), each small enough to stay in registers under indexing and large enough to be
+// indexed rather than selected element by element. rho_at(dp) returns the density at padded
+// offset dp from the lane's source.
+#define NEAR_ACCUMULATORS float a0 = 0.0f, a12[24], a3[32], a4l[18], a4h[18], a56[32]
+#define NEAR_ARGS a0, a12, a3, a4l, a4h, a56
 template <bool EXACT, class RhoAt>
 __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, const float* tab, bool act, RhoAt&& rho_at,
-                                                              float& a0, float (&a1)[6], float (&a2)[18],
-                                                              float (&a3)[32], float (&a4l)[18], float (&a4h)[18],
-                                                              float (&a5)[24], float (&a6)[8],
+                                                              float& a0, float (&a12)[24], float (&a3)[32],
+                                                              float (&a4l)[18], float (&a4h)[18], float (&a56)[32],
                                                               unsigned& visits) {
 #pragma unroll
     for (int i = 0; i < 32; ++i) {
-        if (i < 6) a1[i] = 0.0f;
-        if (i < 18) a2[i] = a4l[i] = a4h[i] = 0.0f;
-        a3[i] = 0.0f;
-        if (i < 24) a5[i] = 0.0f;
-        if (i < 8) a6[i] = 0.0f;
+        if (i < 24) a12[i] = 0.0f;
+        if (i < 18) a4l[i] = a4h[i] = 0.0f;
+        a3[i] = a56[i] = 0.0f;
     }
     const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
     // Two rays at a time: their steps are independent, so the dependent chain of one (depth,
@@ -528,34 +527,45 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
 #pragma unroll
         for (int h = 0; h < 2; ++h) a0 += step(h, 0);  // every near ray starts inside the source voxel
 #pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 1); a1[ns[h][1].loc] += d; }
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 1); a12[ns[h][1].loc] += d; }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 2); a2[ns[h][2].loc] += d; }
+        for (int h = 0; h < 2; ++h) { const float d = step(h, 2); a12[6 + ns[h][2].loc] += d; }
+        // Every ray crosses shells 0-2; the steps past both rays' lengths (which would deposit 0)
+        // are skipped by wave-uniform branches.
+        const int lmax = max(len[0], len[1]);
+        if (lmax > 3) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 3); a3[ns[h][3].loc] += d; }
+            for (int h = 0; h < 2; ++h) { const float d = step(h, 3); a3[ns[h][3].loc] += d; }
+            if (lmax > 4) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {  // both halves are updated, branch-free, to keep the registers in place
-            const float d = step(h, 4);
-            const int loc = ns[h][4].loc;
-            const bool lo = loc < 18;
-            a4l[lo ? loc : 0] += lo ? d : 0.0f;
-            a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
+                for (int h = 0; h < 2; ++h) {  // both halves are updated, branch-free
+                    const float d = step(h, 4);
+                    const int loc = ns[h][4].loc;
+                    const bool lo = loc < 18;
+                    a4l[lo ? loc : 0] += lo ? d : 0.0f;
+                    a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
+                }
+                if (lmax > 5) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) { const float d = step(h, 5); a56[ns[h][5].loc] += d; }
+                    if (lmax > 6) {
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) { const float d = step(h, 6); a56[24 + ns[h][6].loc] += d; }
+                    }
+                }
+            }
         }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 5); a5[ns[h][5].loc] += d; }
-#pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 6); a6[ns[h][6].loc] += d; }
     }
 }
 
 #define NEAR_BIN(q)                                                                                      \
     (kShellMap.shell[q] == 0   ? a0                                                                     \
-     : kShellMap.shell[q] == 1 ? a1[kShellMap.local[q]]                                                 \
-     : kShellMap.shell[q] == 2 ? a2[kShellMap.local[q]]                                                 \
+     : kShellMap.shell[q] == 1 ? a12[kShellMap.local[q]]                                                \
+     : kShellMap.shell[q] == 2 ? a12[6 + kShellMap.local[q]]                                            \
      : kShellMap.shell[q] == 3 ? a3[kShellMap.local[q]]                                                 \
      : kShellMap.shell[q] == 4 ? (kShellMap.local[q] < 18 ? a4l[kShellMap.local[q]] : a4h[kShellMap.local[q] - 18]) \
-     : kShellMap.shell[q] == 5 ? a5[kShellMap.local[q]]                                                 \
-                               : a6[kShellMap.local[q]])
+     : kShellMap.shell[q] == 5 ? a56[kShellMap.local[q]]                                                \
+                               : a56[24 + kShellMap.local[q]])
 
 // Near block over row segments: one wave = 64 consecutive sources of a row (density reads are
 // uniform base + lane). Writes the 125 weights of every source to its own planes and to its
