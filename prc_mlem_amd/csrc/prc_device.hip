@@ -791,24 +791,32 @@ __global__ __launch_bounds__(256) void k_classify(const Vol* __restrict__ vols, 
     if (in) cls[v] = act ? (signed char)(uni ? id : -1) : (signed char)-2;
 }
 
-// The march list of the memo: marched sources and template representatives, appended per wave
-// (the order only changes which lanes march together). Also counts the uniform sources.
-__global__ __launch_bounds__(256) void k_memo_list(int n, const signed char* __restrict__ cls,
-                                                    const int* __restrict__ rep, int2* __restrict__ list,
-                                                    int* __restrict__ ctr) {
+// The march list of the memo: marched sources and template representatives, appended per block
+// (one atomic per block; the order only changes which lanes march together). Also counts the
+// uniform sources.
+constexpr int kMemoBlock = 1024;
+__global__ __launch_bounds__(kMemoBlock) void k_memo_list(int n, const signed char* __restrict__ cls,
+                                                          const int* __restrict__ rep, int2* __restrict__ list,
+                                                          int* __restrict__ ctr) {
+    __shared__ int wt[kMemoBlock / kWave], wu[kMemoBlock / kWave], base;
     const long long vl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & (kWave - 1);
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     const int v = (int)min(vl, (long long)n - 1);
     const int c = vl < n ? (int)cls[v] : -2;
     const bool take = c == -1 || (c >= 0 && rep[c] == v);
     const unsigned long long b = __ballot(take), u = __ballot(c >= 0);
-    int base = 0;
-    if (lane == 0) {
-        if (b) base = atomicAdd(ctr, __popcll(b));
-        if (u) atomicAdd(ctr + 1, __popcll(u));
+    if (lane == 0) wt[wv] = __popcll(b), wu[wv] = __popcll(u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0, us = 0;
+        for (int k = 0; k < kMemoBlock / kWave; ++k) t += wt[k], us += wu[k];
+        base = t ? atomicAdd(ctr, t) : 0;
+        if (us) atomicAdd(ctr + 1, us);
     }
-    base = __shfl(base, 0);
-    if (take) list[base + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
+    __syncthreads();
+    int off = base;
+    for (int k = 0; k < wv; ++k) off += wt[k];
+    if (take) list[off + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
 }
 
 __device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
@@ -1021,7 +1029,7 @@ constexpr int kListSlack = 64;
 
 // Per segment: the runs' offsets relative to the segment, its total entries (aligned) and its
 // planes (the list length, aligned).
-__global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs) {
+__global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs, int* cnt_t) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
     const int unit = blockIdx.x * blockDim.x + threadIdx.x;
     if (unit >= units) return;
@@ -1033,6 +1041,7 @@ __global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes,
         RowInfo* e = V.rinfo + (size_t)unit * R1 + p;
         const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
         e->cnt = cnt;
+        cnt_t[(size_t)p * units + unit] = cnt;  // ray-major copy of the counts
         e->off = off;
         e->xa = x0;
         e->len = w;
@@ -1044,35 +1053,61 @@ __global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes,
     runs[unit].planes = planes;
 }
 
-// Exclusive prefix sums of the per-segment entries and list lengths by one block, 1024 segments at
-// a time in a fixed order; total_a and total_b receive the grand totals.
-__global__ __launch_bounds__(1024) void k_scan_units(const long long* __restrict__ a, const long long* __restrict__ b,
-                                                       long long* __restrict__ a_base, long long* __restrict__ b_base,
-                                                       int n, long long* __restrict__ total_a,
-                                                       long long* __restrict__ total_b) {
-    __shared__ long long wa[16], wb[16];
-    __shared__ long long carry[2];
-    const int t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
-    if (t == 0) carry[0] = carry[1] = 0;
-    __syncthreads();
-    for (int base = 0; base < n; base += 1024) {
-        const int i = base + t;
-        const long long va = i < n ? a[i] : 0, vb = i < n ? b[i] : 0;
-        long long sa = va, sb = vb;  // inclusive scan within the wave
-        for (int o = 1; o < kWave; o <<= 1) {
-            const long long xa = __shfl_up(sa, o), xb = __shfl_up(sb, o);
-            if (lane >= o) sa += xa, sb += xb;
-        }
-        if (lane == kWave - 1) wa[wv] = sa, wb[wv] = sb;
-        __syncthreads();
-        long long pa = carry[0], pb = carry[1];
-        for (int k = 0; k < wv; ++k) pa += wa[k], pb += wb[k];
-        if (i < n) a_base[i] = pa + sa - va, b_base[i] = pb + sb - vb;
-        __syncthreads();
-        if (t == 1023) carry[0] = pa + sa, carry[1] = pb + sb;  // values past n are zeros
-        __syncthreads();
+// Exclusive prefix sums of the per-segment entries and list lengths, in three launches: the sums of
+// 1024-segment tiles, their scan by one block, and the scan inside each tile from its base.
+constexpr int kScanTile = 1024;
+
+// Exclusive scan of (va, vb) over the block's 1024 threads; returns the tile totals in ta, tb.
+__device__ inline void block_scan2(long long va, long long vb, long long& ea, long long& eb, long long& ta,
+                                   long long& tb) {
+    __shared__ long long wa[kScanTile / kWave], wb[kScanTile / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    long long sa = va, sb = vb;  // inclusive scan within the wave
+    for (int o = 1; o < kWave; o <<= 1) {
+        const long long xa = __shfl_up(sa, o), xb = __shfl_up(sb, o);
+        if (lane >= o) sa += xa, sb += xb;
     }
-    if (t == 0) *total_a = carry[0], *total_b = carry[1];
+    if (lane == kWave - 1) wa[wv] = sa, wb[wv] = sb;
+    __syncthreads();
+    long long pa = 0, pb = 0;
+    ta = tb = 0;
+    for (int k = 0; k < kScanTile / kWave; ++k) {
+        if (k < wv) pa += wa[k], pb += wb[k];
+        ta += wa[k], tb += wb[k];
+    }
+    ea = pa + sa - va, eb = pb + sb - vb;
+}
+
+__global__ __launch_bounds__(kScanTile) void k_tile_sums(const long long* __restrict__ a, const long long* __restrict__ b,
+                                                          int n, long long* __restrict__ tile_a,
+                                                          long long* __restrict__ tile_b) {
+    const int i = blockIdx.x * kScanTile + threadIdx.x;
+    long long ea, eb, ta, tb;
+    block_scan2(i < n ? a[i] : 0, i < n ? b[i] : 0, ea, eb, ta, tb);
+    if (threadIdx.x == 0) tile_a[blockIdx.x] = ta, tile_b[blockIdx.x] = tb;
+}
+
+// One block: exclusive bases of up to kScanTile tiles, in place; the grand totals.
+__global__ __launch_bounds__(kScanTile) void k_scan_tiles(long long* __restrict__ tile_a, long long* __restrict__ tile_b,
+                                                           int tiles, long long* __restrict__ total_a,
+                                                           long long* __restrict__ total_b) {
+    const int t = threadIdx.x;
+    long long ea, eb, ta, tb;
+    block_scan2(t < tiles ? tile_a[t] : 0, t < tiles ? tile_b[t] : 0, ea, eb, ta, tb);
+    if (t < tiles) tile_a[t] = ea, tile_b[t] = eb;
+    if (t == 0) *total_a = ta, *total_b = tb;
+}
+
+__global__ __launch_bounds__(kScanTile) void k_scan_in_tiles(const long long* __restrict__ a,
+                                                              const long long* __restrict__ b, int n,
+                                                              const long long* __restrict__ tile_a,
+                                                              const long long* __restrict__ tile_b,
+                                                              long long* __restrict__ a_base,
+                                                              long long* __restrict__ b_base) {
+    const int i = blockIdx.x * kScanTile + threadIdx.x;
+    long long ea, eb, ta, tb;
+    block_scan2(i < n ? a[i] : 0, i < n ? b[i] : 0, ea, eb, ta, tb);
+    if (i < n) a_base[i] = tile_a[blockIdx.x] + ea, b_base[i] = tile_b[blockIdx.x] + eb;
 }
 
 // Absolute offsets: every run moves by its segment's base; the segment's plane sequence and list.
@@ -1103,11 +1138,13 @@ __global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
 
 // Forward bound per (segment, ray): one past the last step at which some source segment that
 // stores that step reaches this segment's x range (an upper bound of where deposits land).
-__global__ void k_forward_bounds(Vol V, const int* kglob) {
-    const int R1 = V.n_rays + 1, nseg = V.nseg;
+// Threads take (ray, segment) with the segment fastest, so neighbouring threads share the ray's
+// steps and read neighbouring entries of the ray-major counts.
+__global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int* __restrict__ cnt_t) {
+    const int R1 = V.n_rays + 1, nseg = V.nseg, units = V.ny * V.nz * nseg;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)V.ny * V.nz * nseg * R1) return;
-    const int unit = (int)(id / R1), p = (int)(id % R1);
+    if (id >= (long long)units * R1) return;
+    const int unit = (int)(id % units), p = (int)(id / units);
     const int row = unit / nseg, seg = unit % nseg;
     const int tw = segment_width(gload(V.row_len + row), seg);
     int kt = 0;
@@ -1124,14 +1161,14 @@ __global__ void k_forward_bounds(Vol V, const int* kglob) {
             for (int ss = 0; ss * kWave < slen; ++ss) {
                 const int sx0 = sxa + kWave * ss, sw = segment_width(slen, ss);
                 if (sx0 + d.x < tx0 + tw && tx0 < sx0 + sw + d.x &&
-                    k - kf < gload(V.rinfo + ((size_t)srow * nseg + ss) * R1 + p).cnt) {
+                    k - kf < cnt_t[(size_t)p * units + srow * nseg + ss]) {
                     kt = k + 1;
                     break;
                 }
             }
         }
     }
-    V.rinfo[id].kt = kt;
+    V.rinfo[(size_t)unit * R1 + p].kt = kt;
 }
 
 // Target-major layout: segment t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
@@ -1173,7 +1210,8 @@ __global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base,
 }
 
 // Shift lists: the entry of step k of ray p's run is sign * D(p, kfirst(p) + k), segment by segment
-// in plane order; the alignment padding of each list is zero. One wave per segment.
+// in plane order; the alignment padding of each list is zero. One wave per segment: the wave
+// reads the runs' counts 64 rays at a time and places each run by a prefix sum over its lanes.
 __global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __restrict__ info,
                                                       const PlaneRun* __restrict__ runs, int sign,
                                                       int* __restrict__ list) {
@@ -1184,12 +1222,25 @@ __global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __res
     const PlaneRun r = runs[unit];
     if (r.planes == 0) return;
     int* out = list + r.dbase;
-    int pos = 0;
-    for (int p = 0; p < R1; ++p) {
-        const int cnt = info[(size_t)unit * R1 + p].cnt;
-        const int* src = V.plin + (size_t)p * V.LP + gload(V.kfirst + p);
-        for (int k = lane; k < cnt; k += kWave) out[pos + k] = sign * src[k];
-        pos += cnt;
+    int pos = 0;  // planes of the rays before this chunk
+    for (int p0 = 0; p0 < R1; p0 += kWave) {
+        const int p = p0 + lane;
+        const int cnt = p < R1 ? info[(size_t)unit * R1 + p].cnt : 0;
+        const int kf = p < R1 ? gload(V.kfirst + p) : 0;
+        int incl = cnt;  // inclusive prefix of the counts over the lanes
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int x = __shfl_up(incl, o);
+            if (lane >= o) incl += x;
+        }
+        const int first = pos + incl - cnt;
+        const int n = min(kWave, R1 - p0);
+        for (int j = 0; j < n; ++j) {  // ray p0 + j: its count and place from lane j
+            const int c = __shfl(cnt, j), f = __shfl(first, j), k0 = __shfl(kf, j);
+            const int* src = V.plin + (size_t)(p0 + j) * V.LP + k0;
+            for (int k = lane; k < c; k += kWave) out[f + k] = sign * src[k];
+        }
+        pos += __shfl(incl, kWave - 1);
     }
     if (lane < align_up(pos, kListAlign) - pos) out[pos + lane] = 0;
 }
@@ -1778,7 +1829,9 @@ struct prc_projector {
     int* kglob = nullptr;
     long long* row_tot = nullptr;
     long long* row_base = nullptr;
-    long long* unit_planes = nullptr;  // per segment: list length (aligned), then its exclusive prefix
+    long long* unit_planes = nullptr;  // per segment: list length (aligned)
+    long long* tile_sums = nullptr;    // layout scan: sums of 1024-segment tiles (two arrays)
+    int* cnt_t = nullptr;              // [n_rays + 1][segments] source-major counts, ray-major
     long long* list_base = nullptr;
     long long* total = nullptr;  // entries (source, target), list lengths (source, target)
     PlaneRun* sruns = nullptr;  // per segment: plane sequence of each copy
@@ -1794,6 +1847,12 @@ struct prc_projector {
     // that it can overlap a batch that is iterating on other projectors. It first waits for the
     // null stream (solvers) and for the streams of the batches that use this projector.
     hipStream_t stream = nullptr;
+    // The target-major copy must be zero before the near block and the fill write its deposits.
+    // When its buffer is kept from the previous normalisation, a second stream clears it while
+    // the count pass and the layout run (zeroed records that the clear is queued there).
+    hipStream_t zstream = nullptr;
+    hipEvent_t zdone = nullptr;
+    bool zeroed = false;
     std::vector<hipStream_t> users;
     // Pinned staging of the sizes normalise_begin reads back (so the read-back does not block
     // the host): total[4] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
@@ -2049,6 +2108,8 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     A->row_tot = dalloc<long long>(units);
     A->row_base = dalloc<long long>(units);
     A->unit_planes = dalloc<long long>(units);
+    A->tile_sums = dalloc<long long>(2 * kScanTile);
+    A->cnt_t = dalloc<int>(units * R1);
     A->list_base = dalloc<long long>(units);
     A->sruns = dalloc<PlaneRun>(units);
     A->truns = dalloc<PlaneRun>(units);
@@ -2072,6 +2133,19 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
     raw = (void*)dalloc<unsigned char>((size_t)(need + 2 * kWave) * wbytes);
     buf = (unsigned char*)raw + (size_t)kWave * wbytes;
     cap = need;
+}
+
+// row_base / list_base = exclusive prefix sums of row_tot / unit_planes over the segments.
+void scan_units(prc_projector* A, long long* total_a, long long* total_b) {
+    const int units = A->hv.ny * A->hv.nz * A->hv.nseg;
+    const int tiles = (int)blocks_for(units, kScanTile);
+    if (tiles > kScanTile) fail(PRC_ERR_INVALID, "too many row segments for the layout scan");
+    long long* ta = A->tile_sums;
+    long long* tb = A->tile_sums + kScanTile;
+    k_tile_sums<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb);
+    k_scan_tiles<<<1, kScanTile, 0, A->stream>>>(ta, tb, tiles, total_a, total_b);
+    k_scan_in_tiles<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb, A->row_base,
+                                                        A->list_base);
 }
 
 // Shift lists, with kListSlack entries of slack past the last list. Allocated zeroed; entries past
@@ -2098,6 +2172,12 @@ void normalise_begin(prc_projector* A) {
     hipStream_t st = A->stream;
     HIPCHK(hipStreamSynchronize(nullptr));  // solver work on this projector
     for (hipStream_t u : A->users) HIPCHK(hipStreamSynchronize(u));  // batches on this projector
+    A->zeroed = false;
+    if (A->traw) {  // nothing reads the target-major copy now: clear it beside the count pass
+        HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, A->zstream));
+        HIPCHK(hipEventRecord(A->zdone, A->zstream));
+        A->zeroed = true;
+    }
     HIPCHK(hipMemsetAsync(A->rinfo, 0, units * R1 * sizeof(RowInfo), st));
     HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), st));
     HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), st));
@@ -2109,7 +2189,8 @@ void normalise_begin(prc_projector* A) {
         HIPCHK(hipMemsetAsync(keys + 2 * kMaxTemplates, 0, 2 * sizeof(int), st));
         const unsigned nb = blocks_for((long long)A->n, 256);
         k_classify<<<nb, 256, 0, st>>>(A->dv, A->cls, (unsigned*)keys, keys + kMaxTemplates);
-        k_memo_list<<<nb, 256, 0, st>>>((int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
+        k_memo_list<<<blocks_for((long long)A->n, kMemoBlock), kMemoBlock, 0, st>>>(
+            (int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
         HIPCHK(hipGetLastError());
     }
     if (A->wbytes == 2) launch_far<__half>(A, 1);
@@ -2117,15 +2198,13 @@ void normalise_begin(prc_projector* A) {
     {
         ProfScope ps("layout", st);
         const unsigned nu = blocks_for((long long)units, 256), nr = blocks_for((long long)units * R1, 256);
-        k_row_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns);
-        k_scan_units<<<1, 1024, 0, st>>>(A->row_tot, A->unit_planes, A->row_base, A->list_base, (int)units,
-                                         A->total, A->total + 2);
+        k_row_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
+        scan_units(A, A->total, A->total + 2);
         k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
         k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
-        k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob);
+        k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t);
         k_target_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
-        k_scan_units<<<1, 1024, 0, st>>>(A->row_tot, A->unit_planes, A->row_base, A->list_base, (int)units,
-                                         A->total + 1, A->total + 3);
+        scan_units(A, A->total + 1, A->total + 3);
         k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns);
         HIPCHK(hipGetLastError());
     }
@@ -2159,6 +2238,10 @@ void normalise_finish(prc_projector* A) {
     if (A->margin < 0) A->margin = margin;
     else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
     ensure_weights(A->wraw, A->wbuf, A->wcap, total[0], A->wbytes);
+    if (A->zeroed && total[1] > A->tcap) {  // the buffer grows: the cleared one is dropped
+        HIPCHK(hipEventSynchronize(A->zdone));
+        A->zeroed = false;
+    }
     ensure_weights(A->traw, A->tbuf, A->tcap, total[1], A->wbytes);
     if (total[2] > INT_MAX - kListSlack || total[3] > INT_MAX - kListSlack)
         fail(PRC_ERR_INVALID, "the system matrix holds too many planes for 32-bit shift-list offsets");
@@ -2169,7 +2252,8 @@ void normalise_finish(prc_projector* A) {
     A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
     V.wbuf = A->wbuf, V.tbuf = A->tbuf, V.slist = A->slist, V.tlist = A->tlist;
     HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
+    if (A->zeroed) HIPCHK(hipStreamWaitEvent(st, A->zdone, 0));
+    else HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
     {
         const size_t units = (size_t)V.ny * V.nz * V.nseg;
         const unsigned nb = blocks_for((long long)units, 256 / kWave);
@@ -2294,12 +2378,14 @@ void read_counters(prc_projector* A) {
 
 void release(prc_projector* A) {
     if (A->stream) (void)hipStreamSynchronize(A->stream), (void)hipStreamDestroy(A->stream), A->stream = nullptr;
+    if (A->zstream) (void)hipStreamSynchronize(A->zstream), (void)hipStreamDestroy(A->zstream), A->zstream = nullptr;
+    if (A->zdone) (void)hipEventDestroy(A->zdone), A->zdone = nullptr;
     if (A->h_stage) (void)hipHostFree(A->h_stage), A->h_stage = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
-    dfree(A->unit_planes), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
+    dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
     if (A->traw) (void)hipFree(A->traw);
@@ -2648,6 +2734,8 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
         try {
             const double t0 = now_s();
             HIPCHK(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
+            HIPCHK(hipStreamCreateWithFlags(&A->zstream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&A->zdone, hipEventDisableTiming));
             setup_projector(A, host_rho, host_table);
             HIPCHK(hipDeviceSynchronize());  // setup ran on the null stream
             normalise_projector(A);
