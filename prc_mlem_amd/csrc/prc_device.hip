@@ -268,7 +268,7 @@ __device__ inline float march_lookup(const float* tab, float scale, float w) {
     const float u = w * scale;
     const int i = (int)u;
     const float lo = tab[i], hi = tab[i + 1];
-    return fmaf(u - (float)i, hi - lo, lo);
+    return fmaf(__builtin_amdgcn_fractf(u), hi - lo, lo);  // fract(u) = u - i exactly for 0 <= u < 2^23
 }
 
 // Index of the calling wave inside its block, known to the compiler as wave-uniform so that
@@ -432,8 +432,10 @@ __device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_
     const float sn = EXACT ? survival(V.br, V.nb, wn) : march_lookup(tab, V.scale, wn);
     const bool tissue = rho >= V.support_rho;
     const float air = tissue ? 0.0f : st.air + seg;
-    const float dep = (inside && tissue && deposit_here && st.surv > sn) ? st.surv - sn : 0.0f;
-    const bool dies = !inside || (!tissue && air > V.air_cap);
+    // surv - sn > 0 exactly when surv > sn, so the max is the reference's guarded difference.
+    const float dep = (inside && tissue && deposit_here) ? fmaxf(st.surv - sn, 0.0f) : 0.0f;
+    // In tissue air = 0 and air_cap > 0, so the air test needs no tissue test.
+    const bool dies = !inside || air > V.air_cap;
     st.w = dies ? V.cutoff : wn;
     st.surv = dies ? V.s_cut : sn;
     st.air = air;
@@ -479,6 +481,21 @@ __device__ inline bool target_slot(const TargetPair& tp, int lane, long long& id
     const bool hi = lane >= tp.thr;
     idx = (hi ? tp.b1 : tp.b0) + lane;
     return ((tp.ok >> (hi ? 1 : 0)) & 1) != 0;
+}
+
+// One step of a far ray as the fill pass uses it, staged in LDS by the lane that resolved it and
+// read back by every lane (32 bytes: two broadcast reads). The target-major slot of lane l is
+// b0 + l, plus delta for l >= thr (the second segment); lanes below lo_lim or from hi_start on
+// hold the step's plane. b0 and delta are chosen so that a missing segment needs no other case.
+struct FillStep {
+    float seg;
+    int dp;
+    long long b0;
+    int delta, thr, lo_lim, hi_start;
+};
+__device__ inline FillStep fill_step(const FarStep& f, const TargetPair& tp) {
+    const long long b0 = (tp.ok & 1) ? tp.b0 : tp.b1, b1 = (tp.ok & 2) ? tp.b1 : b0;
+    return FillStep{f.seg, f.dp, b0, (int)(b1 - b0), tp.thr, (tp.ok & 1) ? tp.thr : 0, (tp.ok & 2) ? tp.thr : kWave};
 }
 
 // The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into the shell
@@ -819,16 +836,6 @@ __global__ __launch_bounds__(kMemoBlock) void k_memo_list(int n, const signed ch
     if (take) list[off + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
 }
 
-__device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ inline float lane_of(float v, int l) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-}
-__device__ inline long long lane_of(long long v, int l) {
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, l);
-    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
-    return (long long)(((unsigned long long)(unsigned)hi << 32) | lo);
-}
-
 // Far rays, count pass: the number of stored steps of every (segment, ray), i.e. one past the last
 // step at which some source of the segment deposits, less kfirst. The count only sizes the layout:
 // an over-count stores extra zero planes and changes no result. A step deposits (prc.cu:146-163)
@@ -881,7 +888,7 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
                 if (COUNT) visits += (inside && alive) ? 1u : 0u;
                 last = (inside && tissue && alive && s0 + u >= kf) ? s0 + u : last;
                 air = tissue ? 0.0f : air + seg[u];
-                const bool dies = !inside || (!tissue && air > air_cap);
+                const bool dies = !inside || air > air_cap;  // in tissue air = 0 < air_cap
                 w = dies ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
             }
         }
@@ -896,14 +903,16 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
 }
 
 // Far rays, fill pass: writes the weights and finishes W (continuing from the near sum, in the
-// reference's accumulation order). The step table is read 64 steps at a time, one step per lane,
-// and each step's values are broadcast from its lane; the same lane also resolves the step's
-// target row (the target-major plane offset), so a step costs one density load and the two weight
-// stores. COUNT adds the number of stored positive far weights to *ndep.
+// reference's accumulation order). The step table is read 64 steps at a time, one step per lane;
+// the same lane also resolves the step's target row (the target-major plane offset) and stages
+// the step in LDS, from where every lane reads it, so a step costs one density load, two LDS
+// broadcast reads and the two weight stores. COUNT adds the number of stored positive far
+// weights to *ndep.
 template <bool EXACT, class WT, bool COUNT>
 __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                    int n_items, unsigned long long* ndep) {
     __shared__ float tab[kTabLds];
+    __shared__ FillStep staged[kItemsPerBlock][kWave];  // per wave: the current 64 steps
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     const int4 it = items[min(item_id, n_items - 1)];
     const Vol& V = vols[it.x];
@@ -960,13 +969,15 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
                                      x0 + fl.dx, sl - kf);
                 }
             }
+            FillStep* const st_w = staged[wave_in_block()];
+            st_w[lane] = fill_step(fl, tl);  // read back below by every lane of this wave only
             const int nst = min(kWave, send - s0);
             for (int j = 0; j < nst; j += kFarGroup) {
                 float seg[kFarGroup], rv[kFarGroup];
 #pragma unroll
                 for (int u = 0; u < kFarGroup; ++u) {
-                    seg[u] = lane_of(fl.seg, j + u);
-                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + lane_of(fl.dp, j + u)) * 4);
+                    seg[u] = st_w[j + u].seg;
+                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + st_w[j + u].dp) * 4);
                 }
                 // Steps past nst are marched too, without stores: past the row's last stored step
                 // no lane of the row deposits (that is what the count pass bounded), and past the
@@ -993,10 +1004,10 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
                     if (valid) gstore(wp + (size_t)su * len, hv[u]);
                     // Target-major copy: step su of this source is step su of its target. A
                     // deposit implies an active target inside its row's active range.
-                    const TargetPair tp{lane_of(tl.b0, j + u), lane_of(tl.b1, j + u), lane_of(tl.thr, j + u),
-                                        lane_of(tl.ok, j + u)};
-                    long long idx;
-                    if (target_slot(tp, lane, idx) && WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + idx, hv[u]);
+                    const FillStep& q = st_w[j + u];
+                    const bool holds = lane < q.lo_lim || lane >= q.hi_start;
+                    const long long idx = q.b0 + lane + (lane >= q.thr ? q.delta : 0);
+                    if (holds && WeightIO<WT>::to_float(hv[u]) > 0.0f) gstore(tbuf + idx, hv[u]);
                 }
                 s = s0 + j + kFarGroup;
                 if (!__any(M.alive(st))) {
