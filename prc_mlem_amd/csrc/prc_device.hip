@@ -1346,7 +1346,7 @@ __device__ inline void load_shifts(const GLOBAL_AS int* p, int (&d)[G]) {
 // 2P - 1 entries past the list (into the next list or the slack: valid shifts either way).
 // Lanes with x >= len read the next plane's entries; their sums are not used.
 template <class WT, class VT>
-__device__ inline void segment_planes(const WT* wseg, int len, int planes, const int* list, const VT* vec0,
+__device__ __forceinline__ void segment_planes(const WT* wseg, int len, int planes, const int* list, const VT* vec0,
                                       int lane, float (&acc)[PlaneLanes<WT>::E]) {
     using L = PlaneLanes<WT>;
     constexpr int E = L::E, LPP = L::LPP, G = L::G, P = L::P;
@@ -1423,7 +1423,7 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
 
 // out = A g where g already holds f / W (one item; see k_forward).
 template <class WT, class VT>
-__device__ inline void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
+__device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
                                     float* __restrict__ out, int lane) {
     constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
     const int row = it.y, x0 = it.z;
@@ -1472,7 +1472,7 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
 
 // out_s = (sum_t w_st in_t) / W_s on active s (one item; see k_adjoint).
 template <class WT, class VT>
-__device__ inline void adjoint_item(const Vol& V, const int4 it, const VT* __restrict__ in,
+__device__ __forceinline__ void adjoint_item(const Vol& V, const int4 it, const VT* __restrict__ in,
                                     float* __restrict__ out, int lane) {
     constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
     const int row = it.y, x0 = it.z;
