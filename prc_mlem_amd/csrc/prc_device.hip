@@ -135,6 +135,13 @@ struct PlaneRun {       // 16 bytes, one scalar load
     int planes;         // planes of all its rays
     int dbase;          // first entry of its shifts in the shift list
 };
+// Where the target-major planes of one segment for one ray lie (a compact, ray-major copy of the
+// target layout's offsets and counts: neighbouring rows of one ray are neighbouring entries).
+struct TargetRun {      // 16 bytes
+    long long off;      // first entry of the run
+    int cnt;            // planes
+    int pad;
+};
 struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
     int dp;             // padded linear offset of the step's voxel from the source
@@ -187,6 +194,7 @@ struct Vol {            // device descriptor of one volume
     int nseg;           // segment slots per row, ceil(nx / 64)
     RowInfo* rinfo;     // [ny*nz*nseg][n_rays + 1] source-major layout (planes over source x)
     RowInfo* tinfo;     // [ny*nz*nseg][n_rays + 1] target-major layout (planes over target x)
+    const TargetRun* tmaj;  // [n_rays + 1][ny*nz*nseg] the target-major runs, ray-major
     void* wbuf;         // source-major weights (float or __half): the adjoint's stream
     void* tbuf;         // target-major weights: the forward's stream
     const PlaneRun* sruns;  // [ny*nz*nseg] source-major plane sequence of each segment
@@ -474,6 +482,26 @@ __device__ inline TargetPair target_pair(const RowInfo* unit0, int R1, int xa_t,
         const RowInfo e = gload(unit0 + (size_t)(s0 + 1) * R1);
         tp.ok |= k < e.cnt ? 2 : 0;
         tp.b1 = e.off - e.xa + (long long)k * e.len + tx0;
+    }
+    return tp;
+}
+// The same from the ray-major runs: row0 points at segment 0 of row t for the ray; the segments'
+// x ranges follow from the row's active range.
+__device__ inline TargetPair target_pair_run(const TargetRun* row0, int xa_t, int len_t, int tx0, int k) {
+    TargetPair tp{0, 0, kWave, 0};
+    const int rel = tx0 - xa_t;
+    const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);  // floor
+    const int ns = (len_t + kWave - 1) / kWave;
+    tp.thr = kWave * (s0 + 1) - rel;  // 1 .. 64
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int sg = s0 + h;
+        if (sg >= 0 && sg < ns) {
+            const TargetRun e = gload(row0 + sg);
+            const int exa = xa_t + kWave * sg, elen = min(kWave, len_t - kWave * sg);
+            tp.ok |= k < e.cnt ? 1 << h : 0;
+            (h ? tp.b1 : tp.b0) = e.off - exa + (long long)k * elen + tx0;
+        }
     }
     return tp;
 }
@@ -811,7 +839,7 @@ __global__ __launch_bounds__(256) void k_classify(const Vol* __restrict__ vols, 
 // The march list of the memo: marched sources and template representatives, appended per block
 // (one atomic per block; the order only changes which lanes march together). Also counts the
 // uniform sources.
-constexpr int kMemoBlock = 1024;
+constexpr int kMemoBlock = 256;
 __global__ __launch_bounds__(kMemoBlock) void k_memo_list(int n, const signed char* __restrict__ cls,
                                                           const int* __restrict__ rep, int2* __restrict__ list,
                                                           int* __restrict__ ctr) {
@@ -920,10 +948,10 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
     if (item_id >= n_items || it.w <= 0) return;
     const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
     const int ny = V.ny, nz = V.nz, L = V.L, LF = V.LF, n_rays = V.n_rays, R1 = n_rays + 1;
+    const size_t units = (size_t)ny * nz * V.nseg;
     const FarStep* fstep = V.fstep;
     const int* kfirst = V.kfirst;
     const RowInfo* rinfo = V.rinfo;
-    const RowInfo* tinfo = V.tinfo;
     WT* wbuf = (WT*)V.wbuf;
     WT* tbuf = (WT*)V.tbuf;
     const int lane = threadIdx.x & (kWave - 1);
@@ -965,8 +993,8 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
                 const int ty = y + fl.dy, tz = z + fl.dz;
                 if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)nz) {
                     const int t = tz * ny + ty;
-                    tl = target_pair(tinfo + (size_t)t * nseg * R1 + 1 + r, R1, gload(row_xa + t), gload(row_len + t),
-                                     x0 + fl.dx, sl - kf);
+                    tl = target_pair_run(V.tmaj + (size_t)(1 + r) * units + (size_t)t * nseg, gload(row_xa + t),
+                                         gload(row_len + t), x0 + fl.dx, sl - kf);
                 }
             }
             FillStep* const st_w = staged[wave_in_block()];
@@ -1065,14 +1093,17 @@ __global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes,
 }
 
 // Exclusive prefix sums of the per-segment entries and list lengths, in three launches: the sums of
-// 1024-segment tiles, their scan by one block, and the scan inside each tile from its base.
-constexpr int kScanTile = 1024;
+// 256-segment tiles, their scan by one block, and the scan inside each tile from its base. Small
+// blocks: these launches run beside the march kernels of other projectors and must fit the CUs
+// those leave free.
+constexpr int kScanTile = 256;
 
-// Exclusive scan of (va, vb) over the block's 1024 threads; returns the tile totals in ta, tb.
+// Exclusive scan of (va, vb) over the block's kScanTile threads; returns the tile totals in ta, tb.
 __device__ inline void block_scan2(long long va, long long vb, long long& ea, long long& eb, long long& ta,
                                    long long& tb) {
     __shared__ long long wa[kScanTile / kWave], wb[kScanTile / kWave];
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    __syncthreads();  // a previous call's readers are done with wa, wb
     long long sa = va, sb = vb;  // inclusive scan within the wave
     for (int o = 1; o < kWave; o <<= 1) {
         const long long xa = __shfl_up(sa, o), xb = __shfl_up(sb, o);
@@ -1098,15 +1129,19 @@ __global__ __launch_bounds__(kScanTile) void k_tile_sums(const long long* __rest
     if (threadIdx.x == 0) tile_a[blockIdx.x] = ta, tile_b[blockIdx.x] = tb;
 }
 
-// One block: exclusive bases of up to kScanTile tiles, in place; the grand totals.
+// One block: exclusive bases of the tiles, in place, kScanTile at a time; the grand totals.
 __global__ __launch_bounds__(kScanTile) void k_scan_tiles(long long* __restrict__ tile_a, long long* __restrict__ tile_b,
                                                            int tiles, long long* __restrict__ total_a,
                                                            long long* __restrict__ total_b) {
-    const int t = threadIdx.x;
-    long long ea, eb, ta, tb;
-    block_scan2(t < tiles ? tile_a[t] : 0, t < tiles ? tile_b[t] : 0, ea, eb, ta, tb);
-    if (t < tiles) tile_a[t] = ea, tile_b[t] = eb;
-    if (t == 0) *total_a = ta, *total_b = tb;
+    long long ca = 0, cb = 0;  // totals of the chunks before
+    for (int base = 0; base < tiles; base += kScanTile) {
+        const int t = base + (int)threadIdx.x;
+        long long ea, eb, ta, tb;
+        block_scan2(t < tiles ? tile_a[t] : 0, t < tiles ? tile_b[t] : 0, ea, eb, ta, tb);
+        if (t < tiles) tile_a[t] = ca + ea, tile_b[t] = cb + eb;
+        ca += ta, cb += tb;
+    }
+    if (threadIdx.x == 0) *total_a = ca, *total_b = cb;
 }
 
 __global__ __launch_bounds__(kScanTile) void k_scan_in_tiles(const long long* __restrict__ a,
@@ -1211,12 +1246,15 @@ __global__ void k_target_totals(Vol V, long long* unit_tot, long long* unit_plan
 }
 
 __global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base,
-                                 const long long* __restrict__ list_base, PlaneRun* runs) {
+                                 const long long* __restrict__ list_base, PlaneRun* runs, TargetRun* tmaj) {
     const int R1 = V.n_rays + 1;
+    const long long units = (long long)V.ny * V.nz * V.nseg;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= (long long)V.ny * V.nz * V.nseg * R1) return;
+    if (id >= units * R1) return;
     const long long unit = id / R1;
-    V.tinfo[id].off += unit_base[unit];
+    RowInfo& e = V.tinfo[id];
+    e.off += unit_base[unit];
+    tmaj[(id % R1) * units + unit] = TargetRun{e.off, e.cnt, 0};
     if (id % R1 == 0) runs[unit].off = unit_base[unit], runs[unit].dbase = (int)list_base[unit];
 }
 
@@ -1841,8 +1879,9 @@ struct prc_projector {
     long long* row_tot = nullptr;
     long long* row_base = nullptr;
     long long* unit_planes = nullptr;  // per segment: list length (aligned)
-    long long* tile_sums = nullptr;    // layout scan: sums of 1024-segment tiles (two arrays)
+    long long* tile_sums = nullptr;    // layout scan: sums of segment tiles (two arrays)
     int* cnt_t = nullptr;              // [n_rays + 1][segments] source-major counts, ray-major
+    TargetRun* tmaj = nullptr;         // [n_rays + 1][segments] target-major runs, ray-major
     long long* list_base = nullptr;
     long long* total = nullptr;  // entries (source, target), list lengths (source, target)
     PlaneRun* sruns = nullptr;  // per segment: plane sequence of each copy
@@ -2119,14 +2158,15 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     A->row_tot = dalloc<long long>(units);
     A->row_base = dalloc<long long>(units);
     A->unit_planes = dalloc<long long>(units);
-    A->tile_sums = dalloc<long long>(2 * kScanTile);
+    A->tile_sums = dalloc<long long>(2 * blocks_for((long long)units, kScanTile));
     A->cnt_t = dalloc<int>(units * R1);
+    A->tmaj = dalloc<TargetRun>(units * R1);
     A->list_base = dalloc<long long>(units);
     A->sruns = dalloc<PlaneRun>(units);
     A->truns = dalloc<PlaneRun>(units);
     A->total = dalloc<long long>(4);
     V.rinfo = A->rinfo, V.tinfo = A->tinfo;
-    V.sruns = A->sruns, V.truns = A->truns;
+    V.sruns = A->sruns, V.truns = A->truns, V.tmaj = A->tmaj;
     V.wbuf = nullptr, V.tbuf = nullptr, V.slist = nullptr, V.tlist = nullptr;
     A->dv = dalloc<Vol>(1);
     HIPCHK(hipMemcpy(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice));
@@ -2150,9 +2190,8 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
 void scan_units(prc_projector* A, long long* total_a, long long* total_b) {
     const int units = A->hv.ny * A->hv.nz * A->hv.nseg;
     const int tiles = (int)blocks_for(units, kScanTile);
-    if (tiles > kScanTile) fail(PRC_ERR_INVALID, "too many row segments for the layout scan");
     long long* ta = A->tile_sums;
-    long long* tb = A->tile_sums + kScanTile;
+    long long* tb = A->tile_sums + tiles;
     k_tile_sums<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb);
     k_scan_tiles<<<1, kScanTile, 0, A->stream>>>(ta, tb, tiles, total_a, total_b);
     k_scan_in_tiles<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb, A->row_base,
@@ -2216,7 +2255,7 @@ void normalise_begin(prc_projector* A) {
         k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t);
         k_target_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
         scan_units(A, A->total + 1, A->total + 3);
-        k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns);
+        k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns, A->tmaj);
         HIPCHK(hipGetLastError());
     }
     if (!A->h_stage) HIPCHK(hipHostMalloc(&A->h_stage, 4 * sizeof(long long) + (2 + R1) * sizeof(int)));
@@ -2365,6 +2404,17 @@ void audit_layout(prc_projector* A) {
     const long long* totals = (const long long*)A->h_stage;  // entries and list lengths of this layout
     check_copy(ri, A->sruns, A->slist, A->stats.weight_entries, totals[2], 1, "source-major");
     check_copy(ti, A->truns, A->tlist, A->stats.target_entries, totals[3], -1, "target-major");
+    {  // the fill's ray-major copy of the target-major runs
+        std::vector<TargetRun> tm(units * R1);
+        HIPCHK(hipMemcpy(tm.data(), A->tmaj, tm.size() * sizeof(TargetRun), hipMemcpyDeviceToHost));
+        for (size_t u = 0; u < units; ++u)
+            for (int p = 0; p < R1; ++p) {
+                const TargetRun& m = tm[(size_t)p * units + u];
+                const RowInfo& e = ti[u * R1 + p];
+                if (m.off != e.off || m.cnt != e.cnt)
+                    fail(PRC_ERR_INVALID, "audit: ray-major target run of segment %zu ray %d differs", u, p);
+            }
+    }
     auto check_item = [&](const std::vector<RowInfo>& info, const int4& it, const char* what) {
         const int row = it.y, x0 = it.z;
         const int seg = (x0 - A->h_row_xa[row]) / kWave;
@@ -2396,7 +2446,7 @@ void release(prc_projector* A) {
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
-    dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
+    dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->tmaj), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
     if (A->traw) (void)hipFree(A->traw);
