@@ -167,7 +167,8 @@ def main():
         fp32 = dict(vols_per_s=args.steps / t, ms_per_volume=1e3 * t / args.steps, weight_bytes=st["weight_bytes"],
                     weight_entries=st["weight_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
                     n_active=st["n_active"], deposits_far=st["deposits"], n_rows=st["n_rows"],
-                    target_entries=st["target_entries"], near_uniform=st["near_uniform"])
+                    target_entries=st["target_entries"], near_uniform=st["near_uniform"],
+                    weight_planes=st["weight_planes"], target_planes=st["target_planes"])
         est32 = S.estimate()
         log(f"plain fp32: {fp32['vols_per_s']:.3f} vols/s, weights {st['weight_bytes'] / 2**30:.2f} GiB")
         if not args.no_profile:
@@ -221,13 +222,19 @@ def main():
                          batches_in_flight=1)
         st = As[0][0].stats
         mixed.update(weight_bytes_per_volume=st["weight_bytes"], weight_entries=st["weight_entries"],
-                     target_entries=st["target_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"])
+                     target_entries=st["target_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
+                     weight_planes=st["weight_planes"], target_planes=st["target_planes"])
         Bs[0].run(ITERS)
         est_mixed = Bs[0].estimate(0)
         log(f"mixed+batched: {mixed['vols_per_s']:.3f} vols/s ({mixed['vols_per_s_sequential']:.3f} sequential)")
         if not args.no_profile:
+            # Profiled step: the batch's projectors normalise one after another here, so that each
+            # phase's event time is one volume's, uncontended; then the batch iterates.
             prc.profile_enable(True)
-            step_seq()
+            for a in As[0]:
+                a.normalise()
+            Bs[0].run(ITERS)
+            Bs[0].sync()
             mixed["kernels"] = prc.profile_report()
             prc.profile_enable(False)
         if fp32 is not None:
@@ -258,8 +265,10 @@ def main():
         wbytes_t = st0["target_entries"] * b_w * vols   # target-major planes (the forward's stream)
         n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
         b_v = 2 if mixed is not None else 4  # the mixed solver gathers FP16 copies of its iterates
-        per_launch = {"forward": wbytes_t + vols * (b_v * n + 4 * n),
-                      "adjoint": wbytes + vols * (b_v * n + 8 * n_act + n)}
+        # + the planes' shift lists (4 B per plane), read once per launch like the weights
+        lists_s, lists_t = 4 * st0["weight_planes"] * vols, 4 * st0["target_planes"] * vols
+        per_launch = {"forward": wbytes_t + lists_t + vols * (b_v * n + 4 * n),
+                      "adjoint": wbytes + lists_s + vols * (b_v * n + 8 * n_act + n)}
         if name in per_launch:
             mean_s = k[name]["mean_ms"] / 1e3
             achieved = per_launch[name] / mean_s / 1e9

@@ -83,6 +83,8 @@ typedef struct {
     int64_t near_uniform;    /* sources whose near block was copied from a uniform-density template */
     double norm_seconds;     /* wall time of the normalisation build */
     float cutoff_mm;         /* water-equivalent range cutoff (prc.cu:83-91, 587) */
+    int64_t weight_planes;   /* source-major planes: entries of the adjoint's shift list */
+    int64_t target_planes;   /* target-major planes: entries of the forward's shift list */
 } prc_stats;
 
 /* Library identification. */
@@ -107,8 +109,9 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
                          prc_projector** out);
 void prc_projector_destroy(prc_projector* projector);
 /* Run the normalisation pass again on the resident density (same result; reuses the weight
- * buffer). Stream-ordered: returns after one device synchronisation (the buffer size) with
- * the march still queued. For callers that time or repeat the build without re-uploading. */
+ * buffer). Waits for work queued on this projector, runs on the projector's own stream and
+ * returns when the build is complete. For callers that time or repeat the build without
+ * re-uploading. */
 int prc_projector_normalise(prc_projector* projector);
 
 /* Normalisation of several projectors at once, each on its own stream so that their kernels

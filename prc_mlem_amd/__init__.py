@@ -52,7 +52,7 @@ class Stats(C.Structure):
                 ("vrs_near", C.c_int64),
                 ("vrs_far", C.c_int64), ("deposits", C.c_int64), ("near_uniform", C.c_int64),
                 ("norm_seconds", C.c_double),
-                ("cutoff_mm", C.c_float)]
+                ("cutoff_mm", C.c_float), ("weight_planes", C.c_int64), ("target_planes", C.c_int64)]
 
 
 # Every symbol include/prc.h declares, with its ctypes signature.
@@ -201,7 +201,7 @@ class Projector:
         _check(lib().prc_projector_adjoint(self.handle, C.c_void_p(y_ptr), C.c_void_p(out_ptr)))
 
     def normalise(self):
-        """Rebuild the stored weights from the resident density (asynchronous)."""
+        """Rebuild the stored weights from the resident density; returns when the build is done."""
         _check(lib().prc_projector_normalise(self.handle))
 
     @property

@@ -2300,6 +2300,8 @@ void normalise_finish(prc_projector* A) {
     A->stats.weight_entries = total[0];
     A->stats.target_entries = total[1];
     A->stats.weight_bytes = (total[0] + total[1]) * A->wbytes;
+    A->stats.weight_planes = total[2];
+    A->stats.target_planes = total[3];
     V.wbuf = A->wbuf, V.tbuf = A->tbuf, V.slist = A->slist, V.tlist = A->tlist;
     HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
     if (A->zeroed) HIPCHK(hipStreamWaitEvent(st, A->zdone, 0));
