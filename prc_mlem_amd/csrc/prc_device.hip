@@ -1459,6 +1459,19 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
     return -1;
 }
 
+// Schedules of the projector passes: one wave per item in launch order (grid), XCD-local work
+// queues (queue), or launch order with each XCD's blocks renumbered to one contiguous chunk of
+// the item list (chunked: block b runs on XCD b % 8; the j-th block of XCD x takes the j-th
+// block-sized piece of chunk x), so that an XCD's gathers stay in its own L2 without queues.
+enum Sched { kSchedGrid = 0, kSchedQueue = 1, kSchedChunked = 2 };
+constexpr int kSchedCount = 3;
+
+__device__ inline unsigned chunked_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x, x = b % kXcds, j = b / kXcds;
+    const unsigned q = nb / kXcds, r = nb % kXcds;  // XCD x runs q + (x < r) blocks
+    return x * q + min(x, r) + j;
+}
+
 // out = A g where g already holds f / W (one item; see k_forward).
 template <class WT, class VT>
 __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
@@ -1486,15 +1499,16 @@ __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const 
 // input vector carries a margin wide enough for every stored shift, so no gather needs a bounds
 // check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
-// QUEUE selects the schedule: XCD-local work queues, or one wave per item in launch order.
-// The gathered vector is FP32, or FP16 in the mixed form (VT).
-template <class WT, class VT, bool QUEUE>
+// SCHED selects the schedule (enum Sched). The gathered vector is FP32, or FP16 in the mixed
+// form (VT).
+template <class WT, class VT, int SCHED>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const VT* __restrict__ gin,
                                                   float* __restrict__ out, int* __restrict__ queue) {
     const int lane = threadIdx.x & (kWave - 1);
-    if (!QUEUE) {
-        const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (SCHED != kSchedQueue) {
+        const unsigned blk = SCHED == kSchedChunked ? chunked_block() : blockIdx.x;
+        const int i = (int)blk * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
         if (it.w != 0) forward_item<WT, VT>(vols[it.x], it, gin, out, lane);
@@ -1533,13 +1547,14 @@ __device__ __forceinline__ void adjoint_item(const Vol& V, const int4 it, const 
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
 // planes and gather in[] at s + D(p,k) (one shift per plane, margin-padded input).
-template <class WT, class VT, bool QUEUE>
+template <class WT, class VT, int SCHED>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const VT* __restrict__ in,
                                                   float* __restrict__ out, int* __restrict__ queue) {
     const int lane = threadIdx.x & (kWave - 1);
-    if (!QUEUE) {
-        const int i = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (SCHED != kSchedQueue) {
+        const unsigned blk = SCHED == kSchedChunked ? chunked_block() : blockIdx.x;
+        const int i = (int)blk * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
         if (it.w > 0) adjoint_item<WT, VT>(vols[it.x], it, in, out, lane);
@@ -2455,9 +2470,8 @@ void release(prc_projector* A) {
     A->wraw = A->traw = A->wbuf = A->tbuf = nullptr;
 }
 
-// Forward/adjoint over one or more volumes sharing a descriptor table and item lists.
-// Schedules of the projector passes (picked per solver or batch by tune_passes).
-enum Sched { kSchedGrid = 0, kSchedQueue = 1 };
+// Forward/adjoint over one or more volumes sharing a descriptor table and item lists; the
+// schedule (enum Sched) is picked per solver or batch by tune_passes.
 
 // `queue` holds kXcds counters in device memory owned by the caller.
 // The gathered input is FP32, or FP16 (`half_in`, the mixed-form solver's iterates) with FP16
@@ -2465,14 +2479,18 @@ enum Sched { kSchedGrid = 0, kSchedQueue = 1 };
 template <class WT, class VT>
 void launch_forward(const Vol* dv, const int4* items, int n_items, const void* g, float* out, int* queue, int sched,
                     unsigned nb, hipStream_t s) {
-    if (sched == kSchedQueue) k_forward<WT, VT, true><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)g, out, queue);
-    else k_forward<WT, VT, false><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)g, out, queue);
+    const VT* in = (const VT*)g;
+    if (sched == kSchedQueue) k_forward<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else if (sched == kSchedChunked) k_forward<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else k_forward<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
 }
 template <class WT, class VT>
 void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* in, float* out, int* queue, int sched,
                     unsigned nb, hipStream_t s) {
-    if (sched == kSchedQueue) k_adjoint<WT, VT, true><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)in, out, queue);
-    else k_adjoint<WT, VT, false><<<nb, 256, 0, s>>>(dv, items, n_items, (const VT*)in, out, queue);
+    const VT* v = (const VT*)in;
+    if (sched == kSchedQueue) k_adjoint<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    else if (sched == kSchedChunked) k_adjoint<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    else k_adjoint<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
 }
 
 void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const void* g, bool half_in, float* out,
@@ -2585,23 +2603,30 @@ void batch_iteration(const Batch& B) {
 // volume the launch-order grid. PRC_TUNE=0 keeps the defaults.
 struct TuneRecord {
     int vols, wbytes, items_fwd, items_adj;
-    float ms[2][2];  // [forward, adjoint][grid, queue], mean of two launches
+    float ms[2][kSchedCount];  // [forward, adjoint][grid, queue, chunked], mean of two launches
     int choice[2];
 };
 std::vector<TuneRecord> g_tuning;
+const char* const kSchedName[kSchedCount] = {"grid", "queue", "chunked"};
 
+// PRC_TUNE=0 keeps the default schedule; PRC_TUNE=grid|queue|chunked forces one (tests).
 void tune_passes(Batch& B) {
     const char* env = getenv("PRC_TUNE");
     if (env && env[0] == '0') return;
+    for (int sched = 0; env && sched < kSchedCount; ++sched)
+        if (strcmp(env, kSchedName[sched]) == 0) {
+            B.fwd_sched = B.adj_sched = sched;
+            return;
+        }
     const bool prof = g_prof_on;
     g_prof_on = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {{0, 0}, {0, 0}}, {0, 0}};
+    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {{0, 0, 0}, {0, 0, 0}}, {0, 0}};
     try {
         for (int op = 0; op < 2; ++op)
-            for (int sched = 0; sched < 2; ++sched) {
+            for (int sched = 0; sched < kSchedCount; ++sched) {
                 auto launch = [&] {
                     if (op == 0)
                         run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf,
@@ -2627,7 +2652,11 @@ void tune_passes(Batch& B) {
     }
     (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
     g_prof_on = prof;
-    for (int op = 0; op < 2; ++op) r.choice[op] = r.ms[op][kSchedQueue] < r.ms[op][kSchedGrid] ? kSchedQueue : kSchedGrid;
+    for (int op = 0; op < 2; ++op) {
+        r.choice[op] = kSchedGrid;
+        for (int sched = 1; sched < kSchedCount; ++sched)
+            if (r.ms[op][sched] < r.ms[op][r.choice[op]]) r.choice[op] = sched;
+    }
     B.fwd_sched = r.choice[0], B.adj_sched = r.choice[1];
     g_tuning.push_back(r);
 }
@@ -3192,10 +3221,11 @@ int prc_tuning_report(char* json, size_t capacity) {
             char buf[512];
             snprintf(buf, sizeof(buf),
                      "%s{\"volumes\": %d, \"weight_bytes\": %d, \"forward_items\": %d, \"adjoint_items\": %d, "
-                     "\"forward_ms\": {\"grid\": %.4f, \"queue\": %.4f}, \"adjoint_ms\": {\"grid\": %.4f, \"queue\": %.4f}, "
+                     "\"forward_ms\": {\"grid\": %.4f, \"queue\": %.4f, \"chunked\": %.4f}, "
+                     "\"adjoint_ms\": {\"grid\": %.4f, \"queue\": %.4f, \"chunked\": %.4f}, "
                      "\"forward\": \"%s\", \"adjoint\": \"%s\"}",
-                     i ? ", " : "", r.vols, r.wbytes, r.items_fwd, r.items_adj, r.ms[0][0], r.ms[0][1], r.ms[1][0],
-                     r.ms[1][1], r.choice[0] ? "queue" : "grid", r.choice[1] ? "queue" : "grid");
+                     i ? ", " : "", r.vols, r.wbytes, r.items_fwd, r.items_adj, r.ms[0][0], r.ms[0][1], r.ms[0][2],
+                     r.ms[1][0], r.ms[1][1], r.ms[1][2], kSchedName[r.choice[0]], kSchedName[r.choice[1]]);
             s += buf;
         }
         s += "]";
