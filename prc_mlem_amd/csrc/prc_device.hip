@@ -216,6 +216,10 @@ struct Scal {
     double hh, hy;       // Biggs-Andrews dot products
     float eps, lambda, scale, beta, gamma;
     int accel;
+    // Power of two applied to the FP16 copy of the forward's input (mixed form): it puts the
+    // data maximum in [256, 512), so that the copy neither overflows nor loses small values;
+    // the forward's output is scaled back exactly in k_ratio.
+    float hscale;
 };
 
 // Loads and stores through the global address space. Pointers kept in descriptors are generic
@@ -1584,12 +1588,15 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
 
 // g = f / W for the forward (ForwardOp::begin).
 // gh, when given, receives the FP16 copy that the mixed form's forward gathers.
+// gh, when given, receives the FP16 copy (times the volume's hscale) that the mixed form's
+// forward gathers.
 __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
-                         const float* __restrict__ f, float* __restrict__ g, __half* __restrict__ gh) {
+                         const Scal* __restrict__ sc, const float* __restrict__ f, float* __restrict__ g,
+                         __half* __restrict__ gh) {
     ITEM_PROLOGUE
     const float v = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
     g[i] = v;
-    if (gh) gh[i] = __float2half_rn(v);
+    if (gh) gh[i] = __float2half_rn(v * sc[it.x].hscale);
 }
 
 // Biggs-Andrews prediction y = max(f + lambda (f - prev), f / 2) (prc.cu:318-321), fused with
@@ -1605,16 +1612,19 @@ __global__ void k_extrapolate(const Vol* __restrict__ vols, const int4* __restri
     y[i] = yi;
     const float v = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
     g[i] = v;
-    if (gh) gh[i] = __float2half_rn(v);
+    if (gh) gh[i] = __float2half_rn(v * sc[it.x].hscale);
 }
 
-// ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place.
+// ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place. When the forward read
+// the FP16 copy (`scaled`), est is hscale times the forward and is scaled back first (exactly:
+// hscale is a power of two).
 // rh, when given, receives the FP16 copy that the mixed form's adjoint gathers.
 __global__ void k_ratio(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
                         const Scal* __restrict__ sc, const float* __restrict__ data, float* __restrict__ est,
-                        __half* __restrict__ rh) {
+                        __half* __restrict__ rh, int scaled) {
     ITEM_PROLOGUE
-    const float r = gload(V.support + loc) ? data[i] / fmaxf(est[i], sc[it.x].eps) : 0.0f;
+    const float e = scaled ? est[i] / sc[it.x].hscale : est[i];
+    const float r = gload(V.support + loc) ? data[i] / fmaxf(e, sc[it.x].eps) : 0.0f;
     est[i] = r;
     if (rh) rh[i] = __float2half_rn(r);
 }
@@ -2571,14 +2581,14 @@ void batch_iteration(const Batch& B) {
             k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf, B.gh);
             start = B.y;
         } else {
-            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf, B.gh);
+            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.gf, B.gh);
         }
     }
     run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
                 B.fwd_sched, B.stream);
     {
         ProfScope ps("ratio", B.stream);
-        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, B.rh);
+        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, B.rh, B.gh != nullptr);
     }
     run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev,
                 B.queue, B.adj_sched, B.stream);
@@ -2670,21 +2680,25 @@ void batch_rotate(Batch& B) {  // prc.cu:697-700
 }
 
 // Host preprocessing of measured data (prc.cu:598-613): clamp, zero off support, FP64 sum.
-double preprocess_data(const prc_projector* A, const float* host_g, float* out) {
+double preprocess_data(const prc_projector* A, const float* host_g, float* out, float* vmax) {
     double support_sum = 0.0;
+    *vmax = 0.0f;
     for (size_t i = 0; i < A->n; ++i) {
         if (!std::isfinite(host_g[i])) fail(PRC_ERR_INVALID, "Non-finite density or activity at voxel %zu", i);
         const float v = std::max(host_g[i], 0.0f);
         out[i] = A->h_support[i] ? v : 0.0f;
-        if (A->h_support[i]) support_sum += v;
+        if (A->h_support[i]) support_sum += v, *vmax = std::max(*vmax, v);
     }
     if (support_sum == 0.0)
         fail(PRC_ERR_NO_ACTIVITY, "No activity in voxels with density >= %g: nothing to do", (double)A->g.support_rho);
     return support_sum;
 }
 
-Scal initial_scalars(const prc_projector* A, double support_sum, const prc_solve_params& p) {
+Scal initial_scalars(const prc_projector* A, double support_sum, float vmax, const prc_solve_params& p) {
     Scal sc{};
+    int e = 0;
+    (void)std::frexp(vmax, &e);  // vmax = m 2^e, m in [0.5, 1): vmax 2^(9 - e) in [256, 512)
+    sc.hscale = std::ldexp(1.0f, std::min(100, std::max(-100, 9 - e)));
     sc.support_sum = support_sum;
     sc.eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);  // prc.cu:668
     sc.lambda = 0.0f, sc.scale = 1.0f, sc.beta = p.beta, sc.gamma = p.gamma;
@@ -2879,7 +2893,8 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
         float* g = A->scratch_in + A->margin;  // f / W with the forward's gather margin
         {
             ProfScope ps("extrapolate", 0);
-            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
+            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, nullptr, dev_in, g,
+                                                                    nullptr);
             HIPCHK(hipGetLastError());
         }
         run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
@@ -2960,12 +2975,13 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
         check_params(params);
         const size_t n = A->n;
         std::vector<float> g(n);
-        const double support_sum = preprocess_data(A, host_g, g.data());
+        float vmax = 0.0f;
+        const double support_sum = preprocess_data(A, host_g, g.data(), &vmax);
         prc_solver* S = new prc_solver;
         S->A = A;
         S->p = *params;
         S->support_sum = support_sum;
-        S->sc0 = initial_scalars(A, support_sum, *params);
+        S->sc0 = initial_scalars(A, support_sum, vmax, *params);
         try {
             const size_t padded = n + 2 * (size_t)A->margin;
             for (int i = 0; i < 7; ++i) {
@@ -3126,8 +3142,9 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             begin[n] = (int)all.size();
             std::vector<float> g(b->total, 0.0f);
             for (int v = 0; v < n; ++v) {
-                const double ss = preprocess_data(projectors[v], host_g[v], g.data() + b->vols[v].vbase);
-                b->sc0[v] = initial_scalars(projectors[v], ss, params[v]);
+                float vmax = 0.0f;
+                const double ss = preprocess_data(projectors[v], host_g[v], g.data() + b->vols[v].vbase, &vmax);
+                b->sc0[v] = initial_scalars(projectors[v], ss, vmax, params[v]);
             }
             auto own = [&](void* p) {
                 b->owned.push_back(p);

@@ -264,3 +264,18 @@ def test_every_schedule_gives_the_same_correction(prc, phantom, blocks, precisio
             np.testing.assert_array_equal(a, b)
     A1.close()
     A2.close()
+
+
+def test_mixed_form_is_scale_equivariant(prc, phantom):
+    # The mixed solver keeps FP16 copies of its forward input; a per-volume power-of-two scale
+    # keeps them in range, so data 2^13 times larger (values far beyond the FP16 maximum) give
+    # exactly 2^13 times the correction (MLEM with extrapolation is scale-equivariant).
+    rho, x, y, sup = phantom
+    announce("mixed form: data scaled by 2^13")
+    A = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    d = np.where(sup, x * np.float32(64.0), 0).astype(np.float32)
+    r1 = prc.correct(A, d, 4, 0.0, 20.0, True)
+    r2 = prc.correct(A, d * np.float32(8192.0), 4, 0.0, 20.0, True)
+    assert np.isfinite(r2).all() and float(d.max()) * 8192 > 65504
+    np.testing.assert_array_equal(r2, r1 * np.float32(8192.0))
+    A.close()
