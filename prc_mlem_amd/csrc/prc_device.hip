@@ -1588,6 +1588,10 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
 
 // g = f / W for the forward (ForwardOp::begin).
 // gh, when given, receives the FP16 copy that the mixed form's forward gathers.
+// Nonnegative value to FP16, round to nearest, saturated at the largest finite FP16 value: a value
+// past it (a ratio whose forward is almost zero) stays finite in the gathers.
+__device__ inline __half to_half_sat(float v) { return __float2half_rn(fminf(v, 65504.0f)); }
+
 // gh, when given, receives the FP16 copy (times the volume's hscale) that the mixed form's
 // forward gathers.
 __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
@@ -1596,7 +1600,7 @@ __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ 
     ITEM_PROLOGUE
     const float v = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
     g[i] = v;
-    if (gh) gh[i] = __float2half_rn(v * sc[it.x].hscale);
+    if (gh) gh[i] = to_half_sat(v * sc[it.x].hscale);
 }
 
 // Biggs-Andrews prediction y = max(f + lambda (f - prev), f / 2) (prc.cu:318-321), fused with
@@ -1612,7 +1616,7 @@ __global__ void k_extrapolate(const Vol* __restrict__ vols, const int4* __restri
     y[i] = yi;
     const float v = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
     g[i] = v;
-    if (gh) gh[i] = __float2half_rn(v * sc[it.x].hscale);
+    if (gh) gh[i] = to_half_sat(v * sc[it.x].hscale);
 }
 
 // ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place. When the forward read
@@ -1626,7 +1630,7 @@ __global__ void k_ratio(const Vol* __restrict__ vols, const int4* __restrict__ i
     const float e = scaled ? est[i] / sc[it.x].hscale : est[i];
     const float r = gload(V.support + loc) ? data[i] / fmaxf(e, sc[it.x].eps) : 0.0f;
     est[i] = r;
-    if (rh) rh[i] = __float2half_rn(r);
+    if (rh) rh[i] = to_half_sat(r);
 }
 
 // RDP MAP-EM surrogate update (prc.cu:279-309). Writes next and per-item FP64 partial sums.
