@@ -1031,16 +1031,15 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
 #pragma unroll
                 for (int u = 0; u < kFarGroup; ++u) {
                     const int su = s0 + j + u;
-                    if (j + u >= nst) break;
-                    if (su < kf) continue;
-                    if (valid) gstore(wp + (size_t)su * len, hv[u]);
+                    const bool stored = (j + u < nst) & (su >= kf);  // a stored step of the run
                     // Target-major copy: step su of this source is step su of its target. A
                     // deposit implies an active target inside its row's active range. The record
-                    // is read whole and the tests combined without short-circuit branches.
+                    // is read whole and the tests combined without branches (j + u < 64).
                     const FillStep q = st_w[j + u];
                     const bool holds = (lane < q.lo_lim) | (lane >= q.hi_start);
                     const long long idx = q.b0 + lane + (lane >= q.thr ? q.delta : 0);
-                    if (holds & (WeightIO<WT>::to_float(hv[u]) > 0.0f)) gstore(tbuf + idx, hv[u]);
+                    if (stored & valid) gstore(wp + (size_t)su * len, hv[u]);
+                    if (stored & holds & (WeightIO<WT>::to_float(hv[u]) > 0.0f)) gstore(tbuf + idx, hv[u]);
                 }
                 s = s0 + j + kFarGroup;
                 if (!__any(M.alive(st))) {
