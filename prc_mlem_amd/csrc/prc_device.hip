@@ -1371,6 +1371,22 @@ __device__ inline void load_values(const GLOBAL_AS char* p, float (&out)[N]) {
     }
 }
 
+// E weights (16 bytes) at byte offset `off` of the buffer, as floats; sc0 set (see segment_planes).
+template <class WT, int E>
+__device__ inline void load_weights(__amdgpu_buffer_rsrc_t r, unsigned off, float (&out)[E]) {
+    static_assert(E * sizeof(WT) == 16, "one 16-byte load per lane");
+    const v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 1));
+    if constexpr (sizeof(WT) == 2) {
+        const h8 h = __builtin_bit_cast(h8, x);
+#pragma unroll
+        for (int e = 0; e < E; ++e) out[e] = (float)h[e];
+    } else {
+        const f4 f = __builtin_bit_cast(f4, x);
+#pragma unroll
+        for (int e = 0; e < E; ++e) out[e] = f[e];
+    }
+}
+
 template <int G>
 __device__ inline void load_shifts(const GLOBAL_AS int* p, int (&d)[G]) {
 #pragma unroll
@@ -1395,8 +1411,12 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
     const int s = lane / LPP, c = lane % LPP;
 #pragma unroll
     for (int e = 0; e < E; ++e) acc[e] = 0.0f;
-    const GLOBAL_AS char* wq = (const GLOBAL_AS char*)wseg;
-    const size_t group_bytes = (size_t)P * len * sizeof(WT);
+    // The weight planes are read through a buffer descriptor over the segment's sequence (32-bit
+    // offsets: a segment's planes stay far below 2 GB), with cache-policy bit sc0 set on the loads;
+    // both were measured faster on the weight stream than plain global loads, the bit by about 5 %.
+    const __amdgpu_buffer_rsrc_t wrsrc = buffer_of(wseg, 0x7fffffffLL);
+    unsigned wq = 0;  // byte offset of the current group's first plane
+    const unsigned group_bytes = (unsigned)(P * len * (int)sizeof(WT));
     unsigned woff[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) woff[g] = (unsigned)(((s * G + g) * len + c * E) * (int)sizeof(WT));
@@ -1414,7 +1434,7 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
         for (int g = 0; g < G; ++g) {
             unsigned off = woff[g];
             if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
-            load_values<WT, E>(wq + off, w[g]);
+            load_weights<WT, E>(wrsrc, wq + off, w[g]);
             load_values<VT, E>(vb + (vlane + (unsigned)d[g] * (unsigned)sizeof(VT)), v[g]);
         }
         if (!full) {
