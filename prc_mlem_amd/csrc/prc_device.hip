@@ -1324,49 +1324,47 @@ struct PlaneLanes {
     static_assert(2 * P <= kListSlack, "the shift prefetch of a last group reads up to 2P - 1 past a list");
 };
 
-// Byte offsets of the streaming loads are 32-bit per-lane values over a wave-uniform 64-bit base
-// (the global "saddr" form). Gathers address the vector through a base moved back by kGatherBias
-// bytes, so that the lane's offset (bias + (x + shift) * element size) is nonnegative for shifts
-// of either sign; shifts stay below 2^29 voxels in magnitude (a volume's density fits in 2^31
-// bytes), so the offset fits in 32 bits.
+// The streaming loads are buffer loads: a wave-uniform descriptor and a 32-bit per-lane byte
+// offset. Gathers address the vector through a base moved back by kGatherBias bytes, so that the
+// lane's offset (bias + (x + shift) * element size) is nonnegative for shifts of either sign;
+// shifts stay below 2^29 voxels in magnitude (a volume's density fits in 2^31 bytes), so the
+// offset fits in 32 bits.
 constexpr unsigned kGatherBias = 0x80000000u;
 
-// 16 (or 8) bytes at any address aligned to the element type: the device reads them unaligned.
-typedef int v4i_u __attribute__((ext_vector_type(4), aligned(2)));
-typedef int v2i_u __attribute__((ext_vector_type(2), aligned(2)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 
-// N consecutive values of T (float or __half) at p, as floats.
+// N consecutive values of T (float or __half) at byte offset `off` of the buffer, as floats (8, 16
+// or 32 bytes; the offset need only be aligned to T: the device reads them unaligned).
 template <class T, int N>
-__device__ inline void load_values(const GLOBAL_AS char* p, float (&out)[N]) {
+__device__ inline void load_gather(__amdgpu_buffer_rsrc_t r, unsigned off, float (&out)[N]) {
     constexpr int bytes = N * (int)sizeof(T);
     if constexpr (bytes % 16 == 0) {
 #pragma unroll
         for (int h = 0; h < bytes / 16; ++h) {
-            const v4i_u r = *(const GLOBAL_AS v4i_u*)(p + 16 * h);
+            const v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 16 * h), 0, 0));
             if constexpr (sizeof(T) == 2) {
-                const h8 x = __builtin_bit_cast(h8, r);
+                const h8 v = __builtin_bit_cast(h8, x);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) out[8 * h + e] = (float)x[e];
+                for (int e = 0; e < 8; ++e) out[8 * h + e] = (float)v[e];
             } else {
-                const f4 x = __builtin_bit_cast(f4, r);
+                const f4 v = __builtin_bit_cast(f4, x);
 #pragma unroll
-                for (int e = 0; e < 4; ++e) out[4 * h + e] = x[e];
+                for (int e = 0; e < 4; ++e) out[4 * h + e] = v[e];
             }
         }
     } else {
         static_assert(bytes == 8, "loads of 8 or a multiple of 16 bytes");
-        const v2i_u r = *(const GLOBAL_AS v2i_u*)p;
+        const v2i x = __builtin_bit_cast(v2i, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
         if constexpr (sizeof(T) == 2) {
-            const h4 x = __builtin_bit_cast(h4, r);
+            const h4 v = __builtin_bit_cast(h4, x);
 #pragma unroll
-            for (int e = 0; e < 4; ++e) out[e] = (float)x[e];
+            for (int e = 0; e < 4; ++e) out[e] = (float)v[e];
         } else {
-            const f2 x = __builtin_bit_cast(f2, r);
-            out[0] = x[0], out[1] = x[1];
+            const f2 v = __builtin_bit_cast(f2, x);
+            out[0] = v[0], out[1] = v[1];
         }
     }
 }
@@ -1420,7 +1418,10 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
     unsigned woff[G];
 #pragma unroll
     for (int g = 0; g < G; ++g) woff[g] = (unsigned)(((s * G + g) * len + c * E) * (int)sizeof(WT));
-    const GLOBAL_AS char* vb = (const GLOBAL_AS char*)vec0 - kGatherBias;
+    // Gathers through a buffer descriptor whose base is moved back by kGatherBias bytes (only
+    // base + offset is ever dereferenced, always inside the vector's allocation).
+    const __amdgpu_buffer_rsrc_t grsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)((const char*)vec0 - kGatherBias), (short)0, (int)0xffffffffu, 0x00020000);
     const unsigned vlane = kGatherBias + (unsigned)(c * E * (int)sizeof(VT));
     const GLOBAL_AS int* lp = (const GLOBAL_AS int*)list + s * G;
     int d[G], dn[G];  // this lane's G shifts of the current and of the next group
@@ -1435,7 +1436,7 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
             unsigned off = woff[g];
             if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
             load_weights<WT, E>(wrsrc, wq + off, w[g]);
-            load_values<VT, E>(vb + (vlane + (unsigned)d[g] * (unsigned)sizeof(VT)), v[g]);
+            load_gather<VT, E>(grsrc, vlane + (unsigned)d[g] * (unsigned)sizeof(VT), v[g]);
         }
         if (!full) {
 #pragma unroll
