@@ -530,24 +530,21 @@ __device__ inline FillStep fill_step(const FarStep& f, const TargetPair& tp) {
     return FillStep{f.seg, f.dp, b0, (int)(b1 - b0), tp.thr, (tp.ok & 1) ? tp.thr : 0, (tp.ok & 2) ? tp.thr : kWave};
 }
 
-// The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into the shell
-// registers: arrays of 18 to 32 entries (shells 1-2 and 5-6 share one, shell 4 is split in two
-// halves of 18), each small enough to stay in registers under indexing and large enough to be
-// indexed rather than selected element by element. rho_at(dp) returns the density at padded
-// offset dp from the lane's source.
-#define NEAR_ACCUMULATORS float a0 = 0.0f, a12[24], a3[32], a4l[18], a4h[18], a56[32]
-#define NEAR_ARGS a0, a12, a3, a4l, a4h, a56
+// The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into registers: four
+// arrays of exactly 32 bins, the size of a register tuple, so that none is padded to the next one.
+// Each is small enough to stay in registers under indexing and large enough to be indexed rather
+// than selected element by element; the bin index of a step is wave-uniform. Layout:
+//   a[0] shell 0, a[1..6] shell 1, a[7..24] shell 2, a[25..28] bins 32-35 of shell 4;
+//   b shell 3; c bins 0-31 of shell 4; d[0..23] shell 5, d[24..31] shell 6.
+// rho_at(dp) returns the density at padded offset dp from the lane's source.
+#define NEAR_ACCUMULATORS float na[32], nb[32], nc[32], nd[32]
+#define NEAR_ARGS na, nb, nc, nd
 template <bool EXACT, class RhoAt>
 __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, const float* tab, bool act, RhoAt&& rho_at,
-                                                              float& a0, float (&a12)[24], float (&a3)[32],
-                                                              float (&a4l)[18], float (&a4h)[18], float (&a56)[32],
-                                                              unsigned& visits) {
+                                                              float (&a)[32], float (&b)[32], float (&c)[32],
+                                                              float (&d)[32], unsigned& visits) {
 #pragma unroll
-    for (int i = 0; i < 32; ++i) {
-        if (i < 24) a12[i] = 0.0f;
-        if (i < 18) a4l[i] = a4h[i] = 0.0f;
-        a3[i] = a56[i] = 0.0f;
-    }
+    for (int i = 0; i < 32; ++i) a[i] = b[i] = c[i] = d[i] = 0.0f;
     const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
     // Two rays at a time: their steps are independent, so the dependent chain of one (depth,
     // table lookup, survival) overlaps the other's. Each bin still receives the rays' deposits in
@@ -565,41 +562,46 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
 #pragma unroll
             for (int s = 0; s < 7; ++s) ns[h][s] = gload(nsr + s);
         }
-        // The whole ray stays inside the 5x5x5 block: load its densities up front (the +inf
-        // margin around the padded density keeps every address in bounds).
+        // The whole ray stays inside the 5x5x5 block (the +inf margin around the padded density
+        // keeps every address in bounds). Densities of steps 0-3 are loaded up front, those of
+        // steps 4-6 only for ray pairs that reach them, while step 3 runs.
+        auto load_rho = [&](int s0, int s1) {
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int s = 0; s < 7; ++s) rv[h][s] = (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
+                for (int s = s0; s < s1; ++s) rv[h][s] = (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
+        };
+        load_rho(0, 4);
         Lane st[2] = {M.start(act), M.start(act)};
         auto step = [&](int h, int s) { return march_step<EXACT>(st[h], rv[h][s], ns[h][s].seg, true, M, tab, visits); };
 #pragma unroll
-        for (int h = 0; h < 2; ++h) a0 += step(h, 0);  // every near ray starts inside the source voxel
+        for (int h = 0; h < 2; ++h) a[0] += step(h, 0);  // every near ray starts inside the source voxel
 #pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 1); a12[ns[h][1].loc] += d; }
+        for (int h = 0; h < 2; ++h) { const float dep = step(h, 1); a[1 + ns[h][1].loc] += dep; }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) { const float d = step(h, 2); a12[6 + ns[h][2].loc] += d; }
+        for (int h = 0; h < 2; ++h) { const float dep = step(h, 2); a[7 + ns[h][2].loc] += dep; }
         // Every ray crosses shells 0-2; the steps past both rays' lengths (which would deposit 0)
         // are skipped by wave-uniform branches.
         const int lmax = max(len[0], len[1]);
         if (lmax > 3) {
+            load_rho(4, 7);
 #pragma unroll
-            for (int h = 0; h < 2; ++h) { const float d = step(h, 3); a3[ns[h][3].loc] += d; }
+            for (int h = 0; h < 2; ++h) { const float dep = step(h, 3); b[ns[h][3].loc] += dep; }
             if (lmax > 4) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {  // both halves are updated, branch-free
-                    const float d = step(h, 4);
+                for (int h = 0; h < 2; ++h) {  // both places are updated, branch-free (x + 0 = x)
+                    const float dep = step(h, 4);
                     const int loc = ns[h][4].loc;
-                    const bool lo = loc < 18;
-                    a4l[lo ? loc : 0] += lo ? d : 0.0f;
-                    a4h[lo ? 0 : loc - 18] += lo ? 0.0f : d;
+                    const bool lo = loc < 32;
+                    c[lo ? loc : 0] += lo ? dep : 0.0f;
+                    a[lo ? 25 : loc - 7] += lo ? 0.0f : dep;
                 }
                 if (lmax > 5) {
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) { const float d = step(h, 5); a56[ns[h][5].loc] += d; }
+                    for (int h = 0; h < 2; ++h) { const float dep = step(h, 5); d[ns[h][5].loc] += dep; }
                     if (lmax > 6) {
 #pragma unroll
-                        for (int h = 0; h < 2; ++h) { const float d = step(h, 6); a56[24 + ns[h][6].loc] += d; }
+                        for (int h = 0; h < 2; ++h) { const float dep = step(h, 6); d[24 + ns[h][6].loc] += dep; }
                     }
                 }
             }
@@ -607,14 +609,15 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
     }
 }
 
+// Bin q of the near block in the layout of near_rays.
 #define NEAR_BIN(q)                                                                                      \
-    (kShellMap.shell[q] == 0   ? a0                                                                     \
-     : kShellMap.shell[q] == 1 ? a12[kShellMap.local[q]]                                                \
-     : kShellMap.shell[q] == 2 ? a12[6 + kShellMap.local[q]]                                            \
-     : kShellMap.shell[q] == 3 ? a3[kShellMap.local[q]]                                                 \
-     : kShellMap.shell[q] == 4 ? (kShellMap.local[q] < 18 ? a4l[kShellMap.local[q]] : a4h[kShellMap.local[q] - 18]) \
-     : kShellMap.shell[q] == 5 ? a56[kShellMap.local[q]]                                                \
-                               : a56[24 + kShellMap.local[q]])
+    (kShellMap.shell[q] == 0   ? na[0]                                                                  \
+     : kShellMap.shell[q] == 1 ? na[1 + kShellMap.local[q]]                                             \
+     : kShellMap.shell[q] == 2 ? na[7 + kShellMap.local[q]]                                             \
+     : kShellMap.shell[q] == 3 ? nb[kShellMap.local[q]]                                                 \
+     : kShellMap.shell[q] == 4 ? (kShellMap.local[q] < 32 ? nc[kShellMap.local[q]] : na[kShellMap.local[q] - 7]) \
+     : kShellMap.shell[q] == 5 ? nd[kShellMap.local[q]]                                                 \
+                               : nd[24 + kShellMap.local[q]])
 
 // Near block over row segments: one wave = 64 consecutive sources of a row (density reads are
 // uniform base + lane). Writes the 125 weights of every source to its own planes and to its
@@ -644,13 +647,19 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     WT* wp = (WT*)V.wbuf + ri.off;
     WT* tb = (WT*)V.tbuf;
     const float nscale = (float)V.n_rays / (float)kNearRays;
-    float wacc = 0.0f;
 #pragma unroll
     for (int q = 0; q < kNQ; ++q) {
         const WT h = WeightIO<WT>::store_value(act ? NEAR_BIN(q) * nscale : 0.0f);
+        if (valid) gstore(wp + (size_t)q * ri.len + lane, h);
+    }
+    // W and the target-major copies, from the source's own planes read back (each lane reads its
+    // own stores) in a loop that is not unrolled, so that the bins above stay in registers.
+    float wacc = 0.0f;
+#pragma unroll 1
+    for (int q = 0; q < kNQ; ++q) {
+        const WT h = valid ? gload(wp + (size_t)q * ri.len + lane) : WeightIO<WT>::store_value(0.0f);
         const float wq = WeightIO<WT>::to_float(h);
         if (wq > 0.0f) wacc += wq;
-        if (valid) gstore(wp + (size_t)q * ri.len + lane, h);
         // A deposit implies an active target, inside its row's active range.
         const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
         if (__any(wq > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
@@ -699,40 +708,45 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
     WT* wp = (WT*)V.wbuf + ri.off + (x - ri.xa);
     WT* tb = (WT*)V.tbuf;
     const float nscale = (float)V.n_rays / (float)kNearRays;
+#pragma unroll
+    for (int q = 0; q < kNQ; ++q) gstore(wp + (size_t)q * ri.len, WeightIO<WT>::store_value(NEAR_BIN(q) * nscale));
+    // W, the template copy and the target-major copies, from the source's own planes read back
+    // (each lane reads its own stores) in a loop that is not unrolled, so that the bins above
+    // stay in registers.
     float wacc = 0.0f;
+#pragma unroll 1
+    for (int dzy = 0; dzy < 25; ++dzy) {
+        WT h[5];
+        float wq[5];
 #pragma unroll
-    for (int dz = -2; dz <= 2; ++dz)
-#pragma unroll
-        for (int dy = -2; dy <= 2; ++dy) {
-            // The five targets x-2 .. x+2 of row t lie in at most two of its segments.
-            const int ty = y + dy, tz = z + dz;
-            RowInfo t0{}, t1{};
-            int txa = 0;
-            if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)V.nz) {
-                const int t = tz * ny + ty, tlen = gload(V.row_len + t);
-                txa = gload(V.row_xa + t);
-                const int rel = x - 2 - txa;
-                const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);
-                const RowInfo* u0 = V.tinfo + (size_t)t * nseg * R1;
-                const int ns = (tlen + kWave - 1) / kWave;
-                if (s0 >= 0 && s0 < ns) t0 = gload(u0 + (size_t)s0 * R1);
-                if (s0 + 1 >= 0 && s0 + 1 < ns) t1 = gload(u0 + (size_t)(s0 + 1) * R1);
-            }
-#pragma unroll
-            for (int dx = -2; dx <= 2; ++dx) {
-                const int q = ((dz + 2) * 5 + dy + 2) * 5 + dx + 2;
-                const WT h = WeightIO<WT>::store_value(NEAR_BIN(q) * nscale);
-                const float wq = WeightIO<WT>::to_float(h);
-                if (wq > 0.0f) wacc += wq;
-                gstore(wp + (size_t)q * ri.len, h);
-                // A deposit implies an active target: inside one of the two segments.
-                const int tx = x + dx;
-                const RowInfo& te = (t1.len > 0 && tx >= t1.xa) ? t1 : t0;
-                if (wq > 0.0f && tx >= te.xa && tx < te.xa + te.len && q < te.cnt)
-                    gstore(tb + te.off + (size_t)q * te.len + (tx - te.xa), h);
-                if (e.y >= 0) tmpl[e.y * kNQ + q] = wq;
-            }
+        for (int k = 0; k < 5; ++k) {
+            const int q = dzy * 5 + k;
+            h[k] = gload(wp + (size_t)q * ri.len);
+            wq[k] = WeightIO<WT>::to_float(h[k]);
+            if (wq[k] > 0.0f) wacc += wq[k];
+            if (e.y >= 0) tmpl[e.y * kNQ + q] = wq[k];
         }
+        // The five targets x-2 .. x+2 of row t lie in at most two of its segments.
+        const int ty = y + dzy % 5 - 2, tz = z + dzy / 5 - 2;
+        if ((unsigned)ty >= (unsigned)ny || (unsigned)tz >= (unsigned)V.nz) continue;
+        const int t = tz * ny + ty, tlen = gload(V.row_len + t), txa = gload(V.row_xa + t);
+        const int rel = x - 2 - txa;
+        const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);
+        const RowInfo* u0 = V.tinfo + (size_t)t * nseg * R1;
+        const int ns = (tlen + kWave - 1) / kWave;
+        RowInfo t0{}, t1{};
+        if (s0 >= 0 && s0 < ns) t0 = gload(u0 + (size_t)s0 * R1);
+        if (s0 + 1 >= 0 && s0 + 1 < ns) t1 = gload(u0 + (size_t)(s0 + 1) * R1);
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int q = dzy * 5 + dx + 2;
+            // A deposit implies an active target: inside one of the two segments.
+            const int tx = x + dx;
+            const RowInfo& te = (t1.len > 0 && tx >= t1.xa) ? t1 : t0;
+            if (wq[dx + 2] > 0.0f && tx >= te.xa && tx < te.xa + te.len && q < te.cnt)
+                gstore(tb + te.off + (size_t)q * te.len + (tx - te.xa), h[dx + 2]);
+        }
+    }
     gstore(V.W + v, wacc);
     if (e.y >= 0) tmpl_sum[e.y] = wacc;
 }
