@@ -269,6 +269,10 @@ __device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* base, long long b
 __device__ inline float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
+// Bias of descriptors whose base is moved back by 2^31 bytes from the data they address: an offset
+// of either sign within +-2^31 bytes of the data becomes a nonnegative 32-bit buffer offset. Such
+// descriptors span the whole 32-bit offset range (no range clamp).
+constexpr unsigned kGatherBias = 0x80000000u;
 
 // Table lookup of prc.cu:74-80 (u = w * scale; i = min(int(u), kBins - 1); lo + min(u - i, 1) *
 // (hi - lo)) on the LDS table with its extra entry tab[kBins + 1] = tab[kBins], without the clamps.
@@ -907,17 +911,24 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
     const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
     const bool valid = lane < it.w;
     const bool act = valid && gload(V.support + (size_t)row * V.nx + x0 + lane);
-    // Density reads through a buffer descriptor over the padded allocation: every lane loads every
-    // step of a group, in bounds by the allocation's margin (a group runs only while a lane is alive).
-    const __amdgpu_buffer_rsrc_t rho_rsrc = buffer_of(V.rho_base, V.rho_bytes);
-    const int rho_idx = (int)(V.rho_p - V.rho_base) +
-                        (int)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
+    // Density reads as buffer loads whose address is split into the lane's part (vector offset,
+    // fixed for the kernel) and the step's wave-uniform part (scalar offset, straight from the
+    // scalar step-table load), so a step's address costs no vector instruction. The descriptor's
+    // base is moved back by kGatherBias bytes and the bias is added to the scalar offset, so both
+    // offsets are nonnegative and their sum, the byte offset from the moved base, stays below
+    // 2^32 (the hardware adds them without wrapping). Every lane loads every step of a group, in
+    // bounds by the allocation's margin (a group runs only while a lane is alive).
+    const __amdgpu_buffer_rsrc_t rho_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const char*)V.rho_p - kGatherBias), (short)0, (int)0xffffffffu, 0x00020000);
+    const int rho_lane = (int)(4u * (unsigned)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane));
     const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
     unsigned visits = 0;
     for (int r = 0; r < n_rays; ++r) {
         const int kf = kfirst[1 + r];
         const FarStep* fs = fstep + (size_t)r * LF;
         float w = act ? 0.0f : cutoff, air = 0.0f;
+        // The lane's last step that may deposit, before or after kf: the wave's count is the part
+        // past kf, the same as when only steps from kf on were considered.
         int last = -1;
         for (int s0 = 0; s0 < LF && __any(w < cutoff); s0 += kCountGroup) {
             float seg[kCountGroup], rv[kCountGroup];
@@ -925,18 +936,22 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
             for (int u = 0; u < kCountGroup; ++u) {
                 const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, dp: a scalar load
                 seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + f[1]) * 4);
+                rv[u] = __builtin_bit_cast(
+                    float, __builtin_amdgcn_raw_buffer_load_b32(rho_rsrc, rho_lane, (int)(kGatherBias + 4u * (unsigned)f[1]), 0));
             }
+            int lg = -1;  // within the group: u is an inline constant of the select
 #pragma unroll
             for (int u = 0; u < kCountGroup; ++u) {
                 const float rho = rv[u];
                 const bool inside = rho < INFINITY, tissue = rho >= support_rho, alive = w < cutoff;
                 if (COUNT) visits += (inside && alive) ? 1u : 0u;
-                last = (inside && tissue && alive && s0 + u >= kf) ? s0 + u : last;
+                lg = (inside && tissue && alive) ? u : lg;
                 air = tissue ? 0.0f : air + seg[u];
-                const bool dies = !inside || air > air_cap;  // in tissue air = 0 < air_cap
-                w = dies ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
+                // Death in air; a step onto the +inf halo kills too, as fminf(fmaf(inf, seg, w),
+                // cutoff) = cutoff (for seg = 0 as well: fminf returns the non-NaN operand).
+                w = air > air_cap ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
             }
+            last = lg >= 0 ? s0 + lg : last;
         }
         // One wave per segment: its count is the segment's.
         const int kmax = wave_max(last + 1);
@@ -1343,7 +1358,6 @@ struct PlaneLanes {
 // lane's offset (bias + (x + shift) * element size) is nonnegative for shifts of either sign;
 // shifts stay below 2^29 voxels in magnitude (a volume's density fits in 2^31 bytes), so the
 // offset fits in 32 bits.
-constexpr unsigned kGatherBias = 0x80000000u;
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
