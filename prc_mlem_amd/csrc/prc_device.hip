@@ -180,7 +180,7 @@ struct Vol {            // device descriptor of one volume
     int nx, ny, nz, n_rays;
     float sx, sy, sz, cutoff, support_rho, air_cap, scale;
     int nb, exact, L, LP;
-    int LF;             // step-table stride: L rounded up to kMarchGroup, zero steps past L
+    int LF;             // step-table stride: L + kFarGroup rounded up to kMarchGroup, zero steps past L
     Branch br[3];
     size_t vbase;       // offset of this volume in concatenated voxel vectors
     const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, +inf halo
@@ -260,14 +260,21 @@ template <> struct WeightIO<__half> {
     __device__ static float to_float(__half v) { return __half2float(v); }
 };
 
-// Buffer-resource load of a float at a 32-bit byte offset from a wave-uniform base: the address
-// costs one 32-bit add instead of 64-bit arithmetic. Descriptor word 3 as for gfx9-family chips.
+// Buffer store of one stored weight (the bits as they are) at byte offset voff + soff.
+template <class WT>
+__device__ inline void buffer_store_weight(WT v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(WT) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// Buffer descriptor over `bytes` bytes from a wave-uniform base: an access then costs a 32-bit
+// offset instead of 64-bit address arithmetic, and accesses at offsets past `bytes` are dropped
+// (stores) or read 0 (loads) by the range check. Descriptor word 3 as for gfx9-family chips.
 __device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* base, long long bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)min(bytes, 0x7fffffffLL),
                                              0x00020000);
-}
-__device__ inline float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int byte_off) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
 }
 // Bias of descriptors whose base is moved back by 2^31 bytes from the data they address: an offset
 // of either sign within +-2^31 bytes of the data becomes a nonnegative 32-bit buffer offset. Such
@@ -450,8 +457,10 @@ __device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_
     const float air = tissue ? 0.0f : st.air + seg;
     // surv - sn > 0 exactly when surv > sn, so the max is the reference's guarded difference.
     const float dep = (inside && tissue && deposit_here) ? fmaxf(st.surv - sn, 0.0f) : 0.0f;
-    // In tissue air = 0 and air_cap > 0, so the air test needs no tissue test.
-    const bool dies = !inside || air > V.air_cap;
+    // In tissue air = 0 and air_cap > 0, so the air test needs no tissue test. A step onto the +inf
+    // halo needs no test either: wn = fminf(fmaf(inf, seg, w), cutoff) = cutoff (for seg = 0 too,
+    // fminf returning the non-NaN operand), so sn = S(cutoff) = s_cut, the state of a dead lane.
+    const bool dies = air > V.air_cap;
     st.w = dies ? V.cutoff : wn;
     st.surv = dies ? V.s_cut : sn;
     st.air = air;
@@ -519,19 +528,17 @@ __device__ inline bool target_slot(const TargetPair& tp, int lane, long long& id
     return ((tp.ok >> (hi ? 1 : 0)) & 1) != 0;
 }
 
-// One step of a far ray as the fill pass uses it, staged in LDS by the lane that resolved it and
-// read back by every lane (32 bytes: two broadcast reads). The target-major slot of lane l is
-// b0 + l, plus delta for l >= thr (the second segment); lanes below lo_lim or from hi_start on
-// hold the step's plane. b0 and delta are chosen so that a missing segment needs no other case.
+// The target-major slots of one step of a far ray as the fill pass uses them, staged in LDS by the
+// lane that resolved them and read back by every lane (24 bytes). The slot of lane l is b0 + l,
+// plus delta for l >= thr (the second segment); lanes below lo_lim or from hi_start on hold the
+// step's plane. b0 and delta are chosen so that a missing segment needs no other case.
 struct FillStep {
-    float seg;
-    int dp;
     long long b0;
     int delta, thr, lo_lim, hi_start;
 };
-__device__ inline FillStep fill_step(const FarStep& f, const TargetPair& tp) {
+__device__ inline FillStep fill_step(const TargetPair& tp) {
     const long long b0 = (tp.ok & 1) ? tp.b0 : tp.b1, b1 = (tp.ok & 2) ? tp.b1 : b0;
-    return FillStep{f.seg, f.dp, b0, (int)(b1 - b0), tp.thr, (tp.ok & 1) ? tp.thr : 0, (tp.ok & 2) ? tp.thr : kWave};
+    return FillStep{b0, (int)(b1 - b0), tp.thr, (tp.ok & 1) ? tp.thr : 0, (tp.ok & 2) ? tp.thr : kWave};
 }
 
 // The 2048 near rays of one source per lane (prc.cu:193-204), accumulated into registers: four
@@ -992,17 +999,23 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
     const bool valid = lane < it.w;
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
-    // Density reads through a buffer descriptor over the padded allocation: every lane loads every
-    // step of a group, in bounds by the allocation's margin (a group runs only while a lane is alive).
-    const __amdgpu_buffer_rsrc_t rho_rsrc = buffer_of(V.rho_base, V.rho_bytes);
-    const int rho_idx = (int)(V.rho_p - V.rho_base) +
-                        (int)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
+    // Density reads as in the count pass: the lane's part of the address in the vector offset, the
+    // step's wave-uniform part (from a scalar step-table load) plus kGatherBias in the scalar offset.
+    // Every lane loads every step of a group, in bounds by the allocation's margin (a group runs only
+    // while a lane is alive).
+    const __amdgpu_buffer_rsrc_t rho_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((const char*)V.rho_p - kGatherBias), (short)0, (int)0xffffffffu, 0x00020000);
+    const int rho_lane = (int)(4u * (unsigned)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane));
     const int nseg = V.nseg;
     const size_t unit = (size_t)row * nseg + (x0 - gload(V.row_xa + row)) / kWave;  // this segment
     const int* row_xa = V.row_xa;
     const int* row_len = V.row_len;
     unsigned visits = 0, deposits = 0;
     float wacc = act ? gload(V.W + loc) : 0.0f;
+    const int len = it.w;  // plane width = the segment's width
+    // Byte offset of the lane's entry in a plane of its own run. Lanes past the segment's width get
+    // an offset past the end of any run, so the run descriptor's range check drops their stores.
+    const int wlane = valid ? lane * (int)sizeof(WT) : 0x7ffffff0;
 
     for (int r = 0; r < n_rays; ++r) {
         const int kf = gload(kfirst + 1 + r);
@@ -1010,19 +1023,77 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
         const RowInfo ri = gload(rinfo + unit * R1 + 1 + r);
         const int cnt = ri.cnt;
         if (cnt <= 0) continue;  // no source of this segment deposits along this ray
-        WT* wp = wbuf + ri.off - (long long)kf * ri.len + lane;
-        const int len = it.w;  // plane width = the segment's width
         // March only up to the row's last stored step.
         const int send = kf + cnt;
+        // The run's planes kf .. send - 1 through a descriptor whose base is plane 0's place: the
+        // store of step su goes to scalar offset su * len entries. (The values are wave-uniform;
+        // readfirstlane puts them in scalar registers, where the descriptor must be.)
+        const unsigned long long wb = (unsigned long long)(wbuf + ri.off - (long long)kf * len);
+        const unsigned long long wbu = ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(wb >> 32)) << 32) |
+                                       (unsigned)__builtin_amdgcn_readfirstlane((int)wb);
+        const __amdgpu_buffer_rsrc_t wrsrc = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)wbu, (short)0, __builtin_amdgcn_readfirstlane((int)((size_t)send * len * sizeof(WT))), 0x00020000);
+        // Seg and dp of steps s .. s + kFarGroup - 1 (scalar loads; s + kFarGroup - 1 < LF, the
+        // table's zero steps past L keeping the reads in bounds) and the lanes' densities there.
+        auto load_group = [&](int s, float (&seg)[kFarGroup], float (&rv)[kFarGroup]) __attribute__((always_inline)) {
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {
+                const v2i f = *(const CONST_AS v2i*)(fs + s + u);
+                seg[u] = __builtin_bit_cast(float, f[0]);
+                rv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                      rho_rsrc, rho_lane, (int)(kGatherBias + 4u * (unsigned)f[1]), 0));
+            }
+        };
         Lane st = M.start(act);
-        int s = 0;
-        bool going = __any(act);
-        for (int s0 = 0; s0 < send && going; s0 += kWave) {
-            const int sl = s0 + lane;
+        // Steps 0 .. kf - 1 lie in the near block: marched for the state they leave, without deposits.
+        for (int s0 = 0; s0 < kf; s0 += kFarGroup) {
+            float seg[kFarGroup], rv[kFarGroup];
+            load_group(s0, seg, rv);
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u)
+                if (s0 + u < kf) (void)march_step<EXACT>(st, rv[u], seg[u], false, M, tab, visits);
+        }
+        // Steps kf .. send - 1, in groups of kFarGroup: step kf + i is plane i of the run. A full
+        // group (FULL) stores all of its steps; the last group of the run may hold steps past send,
+        // marched without stores (past the row's last stored step no lane of the row deposits: that
+        // is what the count pass bounded).
+        auto far_group = [&](auto full_c, int c0, int j, int nst, const FillStep* st_w) __attribute__((always_inline)) {
+            constexpr bool FULL = decltype(full_c)::value;
+            float seg[kFarGroup], rv[kFarGroup];
+            load_group(c0 + j, seg, rv);
+            WT hv[kFarGroup];
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {
+                const float dep = march_step<EXACT>(st, rv[u], seg[u], true, M, tab, visits);
+                hv[u] = WeightIO<WT>::store_value(dep);
+                // Deposits are >= 0 (or -0): adding the ones that are not positive changes nothing.
+                const float wv = WeightIO<WT>::to_float(hv[u]);
+                wacc += wv;
+                if (COUNT && wv > 0.0f) ++deposits;
+            }
+            // The group's stores go out after its last density wait, so no wait of this group
+            // waits on a store.
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {
+                if (FULL || j + u < nst) {
+                    buffer_store_weight(hv[u], wrsrc, wlane, (c0 + j + u) * len * (int)sizeof(WT));
+                    // Target-major copy: step su of this source is step su of its target. A deposit
+                    // implies an active target inside its row's active range.
+                    const FillStep q = st_w[j + u];
+                    const bool holds = (lane < q.lo_lim) | (lane >= q.hi_start);
+                    const long long idx = q.b0 + lane + (lane >= q.thr ? q.delta : 0);
+                    if (holds & (WeightIO<WT>::to_float(hv[u]) > 0.0f)) gstore(tbuf + idx, hv[u]);
+                }
+            }
+        };
+        int s = kf;  // steps before s are done
+        bool going = __any(M.alive(st));
+        for (int c0 = kf; c0 < send && going; c0 += kWave) {
+            const int sl = c0 + lane;
             const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, 0, 0, 0, 0, 0};
             // Target-major slots of step sl (its target row's one or two segments).
             TargetPair tl{0, 0, kWave, 0};
-            if (sl >= kf && sl < send) {
+            if (sl < send) {
                 const int ty = y + fl.dy, tz = z + fl.dz;
                 if ((unsigned)ty < (unsigned)ny && (unsigned)tz < (unsigned)nz) {
                     const int t = tz * ny + ty;
@@ -1031,54 +1102,21 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
                 }
             }
             FillStep* const st_w = staged[wave_in_block()];
-            st_w[lane] = fill_step(fl, tl);  // read back below by every lane of this wave only
-            const int nst = min(kWave, send - s0);
+            st_w[lane] = fill_step(tl);  // read back below by every lane of this wave only
+            const int nst = min(kWave, send - c0);
             for (int j = 0; j < nst; j += kFarGroup) {
-                float seg[kFarGroup], rv[kFarGroup];
-#pragma unroll
-                for (int u = 0; u < kFarGroup; ++u) {
-                    seg[u] = st_w[j + u].seg;
-                    rv[u] = buffer_load_f32(rho_rsrc, (rho_idx + st_w[j + u].dp) * 4);
-                }
-                // Steps past nst are marched too, without stores: past the row's last stored step
-                // no lane of the row deposits (that is what the count pass bounded), and past the
-                // table's end every lane is dead (a ray leaves the grid within nx+ny+nz steps).
-                WT hv[kFarGroup];
-#pragma unroll
-                for (int u = 0; u < kFarGroup; ++u) {
-                    const int su = s0 + j + u;
-                    const float dep = march_step<EXACT>(st, rv[u], seg[u], su >= kf, M, tab, visits);
-                    hv[u] = WeightIO<WT>::store_value(dep);
-                    const float wv = WeightIO<WT>::to_float(hv[u]);
-                    if (wv > 0.0f) {
-                        wacc += wv;
-                        if (COUNT) ++deposits;
-                    }
-                }
-                // The group's stores go out after its last density wait, so no wait of this
-                // group waits on a store.
-#pragma unroll
-                for (int u = 0; u < kFarGroup; ++u) {
-                    const int su = s0 + j + u;
-                    const bool stored = (j + u < nst) & (su >= kf);  // a stored step of the run
-                    // Target-major copy: step su of this source is step su of its target. A
-                    // deposit implies an active target inside its row's active range. The record
-                    // is read whole and the tests combined without branches (j + u < 64).
-                    const FillStep q = st_w[j + u];
-                    const bool holds = (lane < q.lo_lim) | (lane >= q.hi_start);
-                    const long long idx = q.b0 + lane + (lane >= q.thr ? q.delta : 0);
-                    if (stored & valid) gstore(wp + (size_t)su * len, hv[u]);
-                    if (stored & holds & (WeightIO<WT>::to_float(hv[u]) > 0.0f)) gstore(tbuf + idx, hv[u]);
-                }
-                s = s0 + j + kFarGroup;
+                if (j + kFarGroup <= nst)
+                    far_group(std::true_type(), c0, j, nst, st_w);
+                else
+                    far_group(std::false_type(), c0, j, nst, st_w);
+                s = c0 + j + kFarGroup;
                 if (!__any(M.alive(st))) {
                     going = false;
                     break;
                 }
             }
         }
-        for (s = max(s, kf); s < send; ++s)
-            if (valid) gstore(wp + (size_t)s * len, WeightIO<WT>::store_value(0.0f));
+        for (; s < send; ++s) buffer_store_weight(WeightIO<WT>::store_value(0.0f), wrsrc, wlane, s * len * (int)sizeof(WT));
     }
     if (act) gstore(V.W + loc, wacc);
     if (COUNT && ndep) {
@@ -2198,7 +2236,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int R = g.n_rays, R1 = R + 1;
     const int L = g.nx + g.ny + g.nz;  // a ray leaves any grid within nx+ny+nz voxel visits
     const int LP = std::max(L, kNQ);
-    const int LF = (int)align_up(L, kMarchGroup);
+    const int LF = (int)align_up(L + kFarGroup, kMarchGroup);  // the fill reads up to kFarGroup - 1 past L
     A->fstep = dalloc<FarStep>((size_t)R * LF);
     HIPCHK(hipMemset(A->fstep, 0, (size_t)R * LF * sizeof(FarStep)));  // steps past L: no move, no length
     A->pstep = dalloc<int4>((size_t)R1 * LP);
