@@ -547,10 +547,11 @@ __device__ inline FillStep fill_step(const TargetPair& tp) {
 // than selected element by element; the bin index of a step is wave-uniform. Layout:
 //   a[0] shell 0, a[1..6] shell 1, a[7..24] shell 2, a[25..28] bins 32-35 of shell 4;
 //   b shell 3; c bins 0-31 of shell 4; d[0..23] shell 5, d[24..31] shell 6.
-// rho_at(dp) returns the density at padded offset dp from the lane's source.
+// rho_at(dp) returns the density at padded offset dp from the lane's source. COUNT adds the voxel
+// visits (a statistic) to `visits`.
 #define NEAR_ACCUMULATORS float na[32], nb[32], nc[32], nd[32]
 #define NEAR_ARGS na, nb, nc, nd
-template <bool EXACT, class RhoAt>
+template <bool EXACT, bool COUNT, class RhoAt>
 __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, const float* tab, bool act, RhoAt&& rho_at,
                                                               float (&a)[32], float (&b)[32], float (&c)[32],
                                                               float (&d)[32], unsigned& visits) {
@@ -560,8 +561,10 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
     // Two rays at a time: their steps are independent, so the dependent chain of one (depth,
     // table lookup, survival) overlaps the other's. Each bin still receives the rays' deposits in
     // ray order (a ray visits a voxel once), so the sums are those of one ray after another.
-    // Steps past a ray's length see +inf density and a zero step entry: they deposit exactly 0
-    // into bin 0 of their shell and count no visit.
+    // Steps past a ray's length have a zero step entry (seg 0, dp 0, bin 0): w stays, so sn = surv
+    // and they deposit exactly 0 into bin 0 of their shell, whatever density they read; an
+    // inactive lane is dead from the start. Only when counting visits do such steps read +inf
+    // (which counts no visit).
     for (int r = 0; r < kNearRays; r += 2) {
         int len[2];
         NearStep ns[2][7];
@@ -580,7 +583,8 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int s = s0; s < s1; ++s) rv[h][s] = (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
+                for (int s = s0; s < s1; ++s)
+                    rv[h][s] = !COUNT ? rho_at(ns[h][s].dp) : (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
         };
         load_rho(0, 4);
         Lane st[2] = {M.start(act), M.start(act)};
@@ -633,7 +637,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
 // Near block over row segments: one wave = 64 consecutive sources of a row (density reads are
 // uniform base + lane). Writes the 125 weights of every source to its own planes and to its
 // targets' planes (bin q of source s is entry q of target s + d_q), and starts W with their sum.
-template <bool EXACT, class WT>
+template <bool EXACT, class WT, bool COUNT>
 __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                int n_items, unsigned long long* vrs) {
     __shared__ float tab[kTabLds];
@@ -650,7 +654,7 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
     const float* rho0 = V.rho_p + ((size_t)(z + 1) * (V.ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1;
     unsigned visits = 0;
     NEAR_ACCUMULATORS;
-    near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rho0 + dp + lane); }, NEAR_ARGS, visits);
+    near_rays<EXACT, COUNT>(V, tab, act, [&](int dp) { return gload(rho0 + dp + lane); }, NEAR_ARGS, visits);
     // Near weights on the scale of one far ray (prc.cu:205-218); W starts with their sum.
     const int R1 = V.n_rays + 1, nseg = V.nseg;
     const int seg = (x0 - gload(V.row_xa + row)) / kWave;
@@ -682,7 +686,7 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
         }
     }
     if (act) gstore(V.W + loc, wacc);
-    if (vrs) {
+    if (COUNT && vrs) {
         const long long tot = wave_sum_ll((long long)visits);
         if (lane == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
     }
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(256) void k_near(const Vol* __restrict__ vols, cons
 // uniform density, plus one representative per uniform density, whose 125 weights and sum
 // become that density's template (every source with the same uniform neighbourhood computes the
 // same weights bit for bit, so the others copy them in k_near_fill).
-template <bool EXACT, class WT>
+template <bool EXACT, class WT, bool COUNT>
 __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols, const int2* __restrict__ list,
                                                     int n_list, float* __restrict__ tmpl,
                                                     float* __restrict__ tmpl_sum, unsigned long long* vrs) {
@@ -708,8 +712,8 @@ __global__ __launch_bounds__(256) void k_near_list(const Vol* __restrict__ vols,
     const float* rb = V.rho_p + ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + x + 1;
     unsigned visits = 0;
     NEAR_ACCUMULATORS;
-    near_rays<EXACT>(V, tab, act, [&](int dp) { return gload(rb + dp); }, NEAR_ARGS, visits);
-    if (vrs) {
+    near_rays<EXACT, COUNT>(V, tab, act, [&](int dp) { return gload(rb + dp); }, NEAR_ARGS, visits);
+    if (COUNT && vrs) {
         const long long tot = wave_sum_ll((long long)visits);
         if ((threadIdx.x & (kWave - 1)) == 0 && tot) atomicAdd(vrs, (unsigned long long)tot);
     }
@@ -2148,19 +2152,28 @@ template <class WT>
 void launch_near(prc_projector* A) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
     ProfScope ps("near", A->stream);
+    // The voxel visits are counted by a projector's first build only.
+    const bool count = !A->counted;
+    unsigned long long* vrs = A->counters;
+    auto near_list = [&](auto exact_c, auto count_c) {
+        constexpr bool E = decltype(exact_c)::value, C = decltype(count_c)::value;
+        k_near_list<E, WT, C><<<blocks_for(A->n_list, 256), 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list,
+                                                                                A->tmpl, A->tmpl_sum, vrs);
+    };
+    auto near_rows = [&](auto exact_c, auto count_c) {
+        constexpr bool E = decltype(exact_c)::value, C = decltype(count_c)::value;
+        k_near<E, WT, C><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, vrs);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
     if (A->memo) {  // march the listed sources, then copy the templates to the uniform ones
-        const unsigned nl = blocks_for(A->n_list, 256);
-        if (A->hv.exact)
-            k_near_list<true, WT><<<nl, 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum,
-                                                             A->counters);
-        else
-            k_near_list<false, WT><<<nl, 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list, A->tmpl, A->tmpl_sum,
-                                                              A->counters);
+        if (A->hv.exact) count ? near_list(T(), T()) : near_list(T(), F());
+        else count ? near_list(F(), T()) : near_list(F(), F());
         k_near_fill<WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
     } else if (A->hv.exact) {
-        k_near<true, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters);
+        count ? near_rows(T(), T()) : near_rows(T(), F());
     } else {
-        k_near<false, WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->counters);
+        count ? near_rows(F(), T()) : near_rows(F(), F());
     }
     HIPCHK(hipGetLastError());
 }
