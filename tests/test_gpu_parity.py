@@ -168,6 +168,27 @@ def test_renormalise_reset_and_rerun_reproduce(prc, phantom):
     np.testing.assert_array_equal(b.estimate(0), s.reset().iterate(3).estimate())
 
 
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+@pytest.mark.parametrize("memo", [True, False])
+@pytest.mark.parametrize("exact", [False, True])
+def test_rebuild_matches_first_build(prc, phantom, precision, memo, exact):
+    # A projector's first build also counts the march statistics; later builds run the march
+    # kernels without the counting (the near march then reads densities without masks): every
+    # stored weight, and so W and both operators, must come out bit for bit the same.
+    rho, x, y, sup = phantom
+    announce(f"rebuild vs first build {precision} memo={memo} exact={exact}")
+    A = prc.Projector(rho, SPACING, "68Ga", 32, precision=precision, near_memo=memo, exact_range=exact)
+    W0, ax0, aty0, st0 = A.W, A.forward(x), A.adjoint(y), A.stats
+    A.normalise()
+    np.testing.assert_array_equal(A.W, W0)
+    np.testing.assert_array_equal(A.forward(x), ax0)
+    np.testing.assert_array_equal(A.adjoint(y), aty0)
+    st1 = A.stats
+    for k in ("vrs_near", "vrs_far", "deposits", "weight_entries", "target_entries"):
+        assert st1[k] == st0[k], k
+    A.close()
+
+
 def test_profiled_run_matches_graph_run(prc, phantom):
     rho, x, y, sup = phantom
     A = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
