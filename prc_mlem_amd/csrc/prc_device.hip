@@ -281,6 +281,27 @@ __device__ inline __amdgpu_buffer_rsrc_t buffer_of(const void* base, long long b
 // descriptors span the whole 32-bit offset range (no range clamp).
 constexpr unsigned kGatherBias = 0x80000000u;
 
+// Density reads of the marches: buffer loads whose address is split into the lane's part (the
+// vector offset, fixed per lane) and the step's wave-uniform part plus kGatherBias (the scalar
+// offset, straight from a scalar step-table load), over a descriptor moved back by kGatherBias
+// bytes. Both offsets are nonnegative and their sum, the byte offset from the moved base, stays
+// below 2^32 (the hardware adds them without wrapping; a volume's padded density fits in 2^31
+// bytes), so a step's address costs no vector instruction.
+struct RhoReader {
+    __amdgpu_buffer_rsrc_t r;
+    int lane_off;
+    // index: the lane's source in the padded density, counted from rho_p.
+    __device__ RhoReader(const float* rho_p, size_t index)
+        : r(__builtin_amdgcn_make_buffer_rsrc((void*)((const char*)rho_p - kGatherBias), (short)0, (int)0xffffffffu,
+                                              0x00020000)),
+          lane_off((int)(4u * (unsigned)index)) {}
+    // Density at padded offset dp (wave-uniform) from the lane's source.
+    __device__ float operator()(int dp) const {
+        return __builtin_bit_cast(float,
+                                  __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, (int)(kGatherBias + 4u * (unsigned)dp), 0));
+    }
+};
+
 // Table lookup of prc.cu:74-80 (u = w * scale; i = min(int(u), kBins - 1); lo + min(u - i, 1) *
 // (hi - lo)) on the LDS table with its extra entry tab[kBins + 1] = tab[kBins], without the clamps.
 // The march only asks for 0 <= w <= cutoff, so u <= kBins to within one rounding: for u < kBins
@@ -922,16 +943,9 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
     const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
     const bool valid = lane < it.w;
     const bool act = valid && gload(V.support + (size_t)row * V.nx + x0 + lane);
-    // Density reads as buffer loads whose address is split into the lane's part (vector offset,
-    // fixed for the kernel) and the step's wave-uniform part (scalar offset, straight from the
-    // scalar step-table load), so a step's address costs no vector instruction. The descriptor's
-    // base is moved back by kGatherBias bytes and the bias is added to the scalar offset, so both
-    // offsets are nonnegative and their sum, the byte offset from the moved base, stays below
-    // 2^32 (the hardware adds them without wrapping). Every lane loads every step of a group, in
-    // bounds by the allocation's margin (a group runs only while a lane is alive).
-    const __amdgpu_buffer_rsrc_t rho_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)((const char*)V.rho_p - kGatherBias), (short)0, (int)0xffffffffu, 0x00020000);
-    const int rho_lane = (int)(4u * (unsigned)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane));
+    // Every lane loads every step of a group, in bounds by the allocation's margin (a group runs
+    // only while a lane is alive).
+    const RhoReader rho_at(V.rho_p, ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
     const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
     unsigned visits = 0;
     for (int r = 0; r < n_rays; ++r) {
@@ -947,8 +961,7 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
             for (int u = 0; u < kCountGroup; ++u) {
                 const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, dp: a scalar load
                 seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = __builtin_bit_cast(
-                    float, __builtin_amdgcn_raw_buffer_load_b32(rho_rsrc, rho_lane, (int)(kGatherBias + 4u * (unsigned)f[1]), 0));
+                rv[u] = rho_at(f[1]);
             }
             int lg = -1;  // within the group: u is an inline constant of the select
 #pragma unroll
@@ -1003,13 +1016,9 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
     const bool valid = lane < it.w;
     const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
     const bool act = valid && gload(V.support + loc);
-    // Density reads as in the count pass: the lane's part of the address in the vector offset, the
-    // step's wave-uniform part (from a scalar step-table load) plus kGatherBias in the scalar offset.
-    // Every lane loads every step of a group, in bounds by the allocation's margin (a group runs only
-    // while a lane is alive).
-    const __amdgpu_buffer_rsrc_t rho_rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)((const char*)V.rho_p - kGatherBias), (short)0, (int)0xffffffffu, 0x00020000);
-    const int rho_lane = (int)(4u * (unsigned)(((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane));
+    // Every lane loads every step of a group, in bounds by the allocation's margin (a group runs
+    // only while a lane is alive).
+    const RhoReader rho_at(V.rho_p, ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
     const int nseg = V.nseg;
     const size_t unit = (size_t)row * nseg + (x0 - gload(V.row_xa + row)) / kWave;  // this segment
     const int* row_xa = V.row_xa;
@@ -1044,8 +1053,7 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
             for (int u = 0; u < kFarGroup; ++u) {
                 const v2i f = *(const CONST_AS v2i*)(fs + s + u);
                 seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                      rho_rsrc, rho_lane, (int)(kGatherBias + 4u * (unsigned)f[1]), 0));
+                rv[u] = rho_at(f[1]);
             }
         };
         Lane st = M.start(act);
