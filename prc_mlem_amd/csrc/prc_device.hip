@@ -192,6 +192,7 @@ struct Vol {            // device descriptor of one volume
     const int* row_xa;  // [ny*nz]
     const int* row_len; // [ny*nz], 0 = no active voxel
     int nseg;           // segment slots per row, ceil(nx / 64)
+    int box[6];         // bounding box of the support: x0, x1, y0, y1, z0, z1 (inclusive; empty: x0 > x1)
     RowInfo* rinfo;     // [ny*nz*nseg][n_rays + 1] source-major layout (planes over source x)
     RowInfo* tinfo;     // [ny*nz*nseg][n_rays + 1] target-major layout (planes over target x)
     const TargetRun* tmaj;  // [n_rays + 1][ny*nz*nseg] the target-major runs, ray-major
@@ -948,10 +949,19 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
     const RhoReader rho_at(V.rho_p, ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
     const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
     unsigned visits = 0;
+    // Outside the support's bounding box, on an axis along which the ray moves away from the box
+    // (or not at all), a lane never reaches a support voxel again (the walk's voxel coordinates are
+    // monotonic), so it cannot deposit: such lanes are ended at the end of a group, which ends the
+    // wave's air tail early. Only the statistics pass marches every step.
+    const int* box = V.box;
+    const int xl = x0 + lane;
     for (int r = 0; r < n_rays; ++r) {
         const int kf = kfirst[1 + r];
         const FarStep* fs = fstep + (size_t)r * LF;
         float w = act ? 0.0f : cutoff, air = 0.0f;
+        // The ray's direction signs: those of its last tabulated step's offsets.
+        const v4i fe = *(const CONST_AS v4i*)(fs + V.L - 1);
+        const int ex = (short)(fe[2] & 0xffff), ey = fe[2] >> 16, ez = (short)(fe[3] & 0xffff);
         // The lane's last step that may deposit, before or after kf: the wave's count is the part
         // past kf, the same as when only steps from kf on were considered.
         int last = -1;
@@ -976,6 +986,14 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
                 w = air > air_cap ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
             }
             last = lg >= 0 ? s0 + lg : last;
+            if (!COUNT) {
+                const v4i fc = *(const CONST_AS v4i*)(fs + s0 + kCountGroup - 1);  // the group's last voxel
+                const int cx = xl + (short)(fc[2] & 0xffff), cy = y + (fc[2] >> 16), cz = z + (short)(fc[3] & 0xffff);
+                const bool away_yz = (cy < box[2] && ey <= 0) || (cy > box[3] && ey >= 0) ||
+                                     (cz < box[4] && ez <= 0) || (cz > box[5] && ez >= 0);
+                const bool away = away_yz || (cx < box[0] && ex <= 0) || (cx > box[1] && ex >= 0);
+                w = away ? cutoff : w;
+            }
         }
         // One wave per segment: its count is the segment's.
         const int kmax = wave_max(last + 1);
@@ -2212,6 +2230,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     std::vector<float> rp(np + 2 * margin, INFINITY);
     A->h_support.assign(n, 0);
     long long n_active = 0;
+    int box[6] = {g.nx, -1, g.ny, -1, g.nz, -1};
     for (int k = 0; k < g.nz; ++k)
         for (int j = 0; j < g.ny; ++j) {
             const size_t src = ((size_t)k * g.ny + j) * g.nx;
@@ -2222,6 +2241,11 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
                 const bool s = r >= g.support_rho;
                 A->h_support[src + i] = s;
                 n_active += s;
+                if (s) {
+                    box[0] = std::min(box[0], i), box[1] = std::max(box[1], i);
+                    box[2] = std::min(box[2], j), box[3] = std::max(box[3], j);
+                    box[4] = std::min(box[4], k), box[5] = std::max(box[5], k);
+                }
             }
         }
     A->stats.n_active = n_active;
@@ -2277,6 +2301,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     for (int i = 0; i < 3; ++i) V.br[i] = i < A->model.n ? A->model.br[i] : Branch{0, 0, 0, 0, 0};
     V.exact = (g.flags & PRC_FLAG_EXACT_RANGE) != 0;
     V.L = L, V.LP = LP, V.LF = LF, V.vbase = 0;
+    std::copy(box, box + 6, V.box);
     V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
     V.rho_base = A->rho_alloc, V.rho_bytes = (long long)(np + 2 * margin) * (long long)sizeof(float);
     V.fstep = A->fstep, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
