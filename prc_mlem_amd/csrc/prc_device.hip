@@ -1541,10 +1541,17 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
 #pragma unroll
                     for (int e = 0; e < E; ++e) w[g][e] = 0.0f;
         }
+        // Zero weights must not carry a non-finite input value into the sum (float vectors come
+        // from callers of the operators). FP16 copies are finite by construction (written in
+        // [0, 65504] by saturation, margins zero), and for finite v, fmaf(0, v, acc) is acc bit for
+        // bit (acc starts at +0 and never becomes -0), so they need no select: one mixed-precision
+        // FMA per entry.
+        constexpr bool kFinite = std::is_same<VT, __half>::value;
 #pragma unroll
         for (int g = 0; g < G; ++g)
 #pragma unroll
-            for (int e = 0; e < E; ++e) acc[e] = w[g][e] != 0.0f ? fmaf(w[g][e], v[g][e], acc[e]) : acc[e];
+            for (int e = 0; e < E; ++e)
+                acc[e] = (kFinite || w[g][e] != 0.0f) ? fmaf(w[g][e], v[g][e], acc[e]) : acc[e];
 #pragma unroll
         for (int g = 0; g < G; ++g) d[g] = dn[g];
     }
