@@ -1465,11 +1465,14 @@ __device__ inline void load_gather(__amdgpu_buffer_rsrc_t r, unsigned off, float
     }
 }
 
-// E weights (16 bytes) at byte offset `off` of the buffer, as floats; sc0 set (see segment_planes).
+// E weights (16 bytes) at byte offset `off` of the buffer, as floats, or zeros unless `keep` (the
+// zeroing is done on the loaded bits, so that each weight stays a plain conversion that the FMA
+// consuming it can absorb); sc0 set (see segment_planes).
 template <class WT, int E>
-__device__ inline void load_weights(__amdgpu_buffer_rsrc_t r, unsigned off, float (&out)[E]) {
+__device__ inline void load_weights(__amdgpu_buffer_rsrc_t r, unsigned off, float (&out)[E], bool keep) {
     static_assert(E * sizeof(WT) == 16, "one 16-byte load per lane");
-    const v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 1));
+    v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 1));
+    if (!keep) x = v4i{0, 0, 0, 0};
     if constexpr (sizeof(WT) == 2) {
         const h8 h = __builtin_bit_cast(h8, x);
 #pragma unroll
@@ -1531,15 +1534,8 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
         for (int g = 0; g < G; ++g) {
             unsigned off = woff[g];
             if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
-            load_weights<WT, E>(wrsrc, wq + off, w[g]);
+            load_weights<WT, E>(wrsrc, wq + off, w[g], full || q + s * G + g < planes);
             load_gather<VT, E>(grsrc, vlane + (unsigned)d[g] * (unsigned)sizeof(VT), v[g]);
-        }
-        if (!full) {
-#pragma unroll
-            for (int g = 0; g < G; ++g)
-                if (q + s * G + g >= planes)
-#pragma unroll
-                    for (int e = 0; e < E; ++e) w[g][e] = 0.0f;
         }
         // Zero weights must not carry a non-finite input value into the sum (float vectors come
         // from callers of the operators). FP16 copies are finite by construction (written in
