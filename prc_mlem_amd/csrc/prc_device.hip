@@ -1563,7 +1563,6 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
 // XCD works on stay in that XCD's L2, and moves on to the next chunks once its own is drained
 // (load balance). The counters are cleared by k_reset_queue before every launch.
 constexpr int kXcds = 8;
-constexpr int kProjBlocks = 2048;  // resident blocks at full occupancy: 256 CUs x 8
 
 __global__ void k_reset_queue(int* queue) {
     if (threadIdx.x < kXcds) queue[threadIdx.x] = 0;
@@ -2643,10 +2642,24 @@ void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* i
     else k_adjoint<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
 }
 
+// Persistent grid of the queue schedule: two blocks (eight waves) per CU, a quarter of the waves
+// that fit. Fewer resident waves keep each XCD's concurrent gathers within fewer rows (measured
+// faster on the mixed passes than a full grid) and leave wave slots to the kernels of another
+// stream running beside the pass.
+unsigned queue_blocks() {
+    static const unsigned nb = [] {
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        return 2u * (unsigned)std::max(cus, 1);
+    }();
+    return nb;
+}
+
 void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const void* g, bool half_in, float* out,
                  int* queue, int sched, hipStream_t s) {
     const unsigned all = blocks_for(n_items, kItemsPerBlock);
-    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
+    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(queue_blocks(), all) : all;
     if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("forward", s);
     if (wbytes == 4) launch_forward<float, float>(dv, items, n_items, g, out, queue, sched, nb, s);
@@ -2658,7 +2671,7 @@ void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, cons
 void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const void* in, bool half_in, float* out,
                  int* queue, int sched, hipStream_t s) {
     const unsigned all = blocks_for(n_items, kItemsPerBlock);
-    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(kProjBlocks, all) : all;
+    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(queue_blocks(), all) : all;
     if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("adjoint", s);
     if (wbytes == 4) launch_adjoint<float, float>(dv, items, n_items, in, out, queue, sched, nb, s);
