@@ -144,7 +144,8 @@ struct TargetRun {      // 16 bytes
 };
 struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
-    int dp;             // padded linear offset of the step's voxel from the source
+    unsigned soff;      // 2^31 + 4 * (padded linear offset of the step's voxel from the source): the
+                        // scalar offset of the step's density load (RhoReader::step)
     short dx, dy, dz;   // the step's voxel relative to the source (the deposit's target)
     short pad;
 };
@@ -297,9 +298,10 @@ struct RhoReader {
                                               0x00020000)),
           lane_off((int)(4u * (unsigned)index)) {}
     // Density at padded offset dp (wave-uniform) from the lane's source.
-    __device__ float operator()(int dp) const {
-        return __builtin_bit_cast(float,
-                                  __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, (int)(kGatherBias + 4u * (unsigned)dp), 0));
+    __device__ float operator()(int dp) const { return step(kGatherBias + 4u * (unsigned)dp); }
+    // The same with the scalar offset kGatherBias + 4 * dp given (a far step's soff).
+    __device__ float step(unsigned soff) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, (int)soff, 0));
     }
 };
 
@@ -410,8 +412,11 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirs
             pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
             plin[(size_t)(1 + id) * V.LP + s] = linear_offset(V, i, j, k);
             const float seg = w.step();
-            fstep[(size_t)id * V.LF + s] = FarStep{seg, (k * nyp + j) * nxp + i, (short)i, (short)j, (short)k, 0};
+            fstep[(size_t)id * V.LF + s] =
+                FarStep{seg, kGatherBias + 4u * (unsigned)((k * nyp + j) * nxp + i), (short)i, (short)j, (short)k, 0};
         }
+        // Steps past L: no move and no length (the source voxel itself).
+        for (int s = V.L; s < V.LF; ++s) fstep[(size_t)id * V.LF + s] = FarStep{0.0f, kGatherBias, 0, 0, 0, 0};
         kfirst[1 + id] = kf < 0 ? V.L : kf;
     } else if (id < V.n_rays + kNearRays) {
         const int r = id - V.n_rays;
@@ -969,9 +974,9 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
             float seg[kCountGroup], rv[kCountGroup];
 #pragma unroll
             for (int u = 0; u < kCountGroup; ++u) {
-                const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, dp: a scalar load
+                const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, soff: a scalar load
                 seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = rho_at(f[1]);
+                rv[u] = rho_at.step((unsigned)f[1]);
             }
             int lg = -1;  // within the group: u is an inline constant of the select
 #pragma unroll
@@ -1071,7 +1076,7 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
             for (int u = 0; u < kFarGroup; ++u) {
                 const v2i f = *(const CONST_AS v2i*)(fs + s + u);
                 seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = rho_at(f[1]);
+                rv[u] = rho_at.step((unsigned)f[1]);
             }
         };
         Lane st = M.start(act);
@@ -1120,7 +1125,7 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
         bool going = __any(M.alive(st));
         for (int c0 = kf; c0 < send && going; c0 += kWave) {
             const int sl = c0 + lane;
-            const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, 0, 0, 0, 0, 0};
+            const FarStep fl = sl < L ? gload(fs + sl) : FarStep{0.0f, kGatherBias, 0, 0, 0, 0};
             // Target-major slots of step sl (its target row's one or two segments).
             TargetPair tl{0, 0, kWave, 0};
             if (sl < send) {
