@@ -204,6 +204,8 @@ struct Vol {            // device descriptor of one volume
     const int* slist;       // source-major shifts +D (the adjoint gathers at s + D)
     const int* tlist;       // target-major shifts -D (the forward gathers at t - D)
     const FarStep* fstep;   // [n_rays][LF]
+    const unsigned* fsoa;   // [n_rays][2][LF]: the steps' seg bits, then their soff (a group's steps
+                            // in two wide scalar loads)
     const int4* pstep;      // [n_rays + 1][LP] (dx, dy, dz) of each step; p = 0 is the near block
     const int* plin;        // [n_rays + 1][LP] linear voxel offset (dz*ny + dy)*nx + dx of each step
     const int* kfirst;      // [n_rays + 1] first stored step
@@ -398,7 +400,8 @@ __device__ inline int linear_offset(const Vol& V, int i, int j, int k) {
     return (int)max(min(v, (long long)INT_MAX), (long long)INT_MIN);
 }
 
-__global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirst, NearStep* nstep, int* nlen) {
+__global__ void k_rays(Vol V, FarStep* fstep, unsigned* fsoa, int4* pstep, int* plin, int* kfirst, NearStep* nstep,
+                       int* nlen) {
     const int id = blockIdx.x * blockDim.x + threadIdx.x;
     const int nxp = V.nx + 2, nyp = V.ny + 2;
     if (id < V.n_rays) {
@@ -412,11 +415,17 @@ __global__ void k_rays(Vol V, FarStep* fstep, int4* pstep, int* plin, int* kfirs
             pstep[(size_t)(1 + id) * V.LP + s] = make_int4(i, j, k, 0);
             plin[(size_t)(1 + id) * V.LP + s] = linear_offset(V, i, j, k);
             const float seg = w.step();
-            fstep[(size_t)id * V.LF + s] =
-                FarStep{seg, kGatherBias + 4u * (unsigned)((k * nyp + j) * nxp + i), (short)i, (short)j, (short)k, 0};
+            const unsigned soff = kGatherBias + 4u * (unsigned)((k * nyp + j) * nxp + i);
+            fstep[(size_t)id * V.LF + s] = FarStep{seg, soff, (short)i, (short)j, (short)k, 0};
+            fsoa[(size_t)id * 2 * V.LF + s] = __float_as_uint(seg);
+            fsoa[(size_t)id * 2 * V.LF + V.LF + s] = soff;
         }
         // Steps past L: no move and no length (the source voxel itself).
-        for (int s = V.L; s < V.LF; ++s) fstep[(size_t)id * V.LF + s] = FarStep{0.0f, kGatherBias, 0, 0, 0, 0};
+        for (int s = V.L; s < V.LF; ++s) {
+            fstep[(size_t)id * V.LF + s] = FarStep{0.0f, kGatherBias, 0, 0, 0, 0};
+            fsoa[(size_t)id * 2 * V.LF + s] = 0u;
+            fsoa[(size_t)id * 2 * V.LF + V.LF + s] = kGatherBias;
+        }
         kfirst[1 + id] = kf < 0 ? V.L : kf;
     } else if (id < V.n_rays + kNearRays) {
         const int r = id - V.n_rays;
@@ -970,13 +979,19 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
         // The lane's last step that may deposit, before or after kf: the wave's count is the part
         // past kf, the same as when only steps from kf on were considered.
         int last = -1;
+        const unsigned* fa = V.fsoa + (size_t)r * 2 * LF;  // the ray's seg bits, then its soffs
         for (int s0 = 0; s0 < LF && __any(w < cutoff); s0 += kCountGroup) {
             float seg[kCountGroup], rv[kCountGroup];
+            // The group's lengths and offsets: 16-byte scalar loads at immediate offsets from two
+            // bases (s0 and LF are multiples of 16, so every load is aligned).
 #pragma unroll
-            for (int u = 0; u < kCountGroup; ++u) {
-                const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);  // seg, soff: a scalar load
-                seg[u] = __builtin_bit_cast(float, f[0]);
-                rv[u] = rho_at.step((unsigned)f[1]);
+            for (int u4 = 0; u4 < kCountGroup; u4 += 4) {
+                const v4i gs = *(const CONST_AS v4i*)(fa + s0 + u4), go = *(const CONST_AS v4i*)(fa + LF + s0 + u4);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    seg[u4 + e] = __int_as_float(gs[e]);  // (hipcc miscompiles __builtin_bit_cast of a vector element)
+                    rv[u4 + e] = rho_at.step((unsigned)go[e]);
+                }
             }
             int lg = -1;  // within the group: u is an inline constant of the select
 #pragma unroll
@@ -1069,13 +1084,14 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
                                        (unsigned)__builtin_amdgcn_readfirstlane((int)wb);
         const __amdgpu_buffer_rsrc_t wrsrc = __builtin_amdgcn_make_buffer_rsrc(
             (void*)wbu, (short)0, __builtin_amdgcn_readfirstlane((int)((size_t)send * len * sizeof(WT))), 0x00020000);
-        // Seg and dp of steps s .. s + kFarGroup - 1 (scalar loads; s + kFarGroup - 1 < LF, the
-        // table's zero steps past L keeping the reads in bounds) and the lanes' densities there.
+        // Seg and soff of steps s .. s + kFarGroup - 1 (scalar loads, one per step: a group starts
+        // at any step, and wide scalar loads need their natural alignment; s + kFarGroup - 1 < LF,
+        // the table's zero steps past L keeping the reads in bounds) and the lanes' densities there.
         auto load_group = [&](int s, float (&seg)[kFarGroup], float (&rv)[kFarGroup]) __attribute__((always_inline)) {
 #pragma unroll
             for (int u = 0; u < kFarGroup; ++u) {
                 const v2i f = *(const CONST_AS v2i*)(fs + s + u);
-                seg[u] = __builtin_bit_cast(float, f[0]);
+                seg[u] = __int_as_float(f[0]);
                 rv[u] = rho_at.step((unsigned)f[1]);
             }
         };
@@ -2023,6 +2039,7 @@ struct prc_projector {
     long long wcap = 0, tcap = 0;  // weight buffer capacities (entries)
     long long margin = -1;         // elements of padding on each side of gathered vectors
     FarStep* fstep = nullptr;
+    unsigned* fsoa = nullptr;
     int4* pstep = nullptr;
     int* plin = nullptr;
     int* kfirst = nullptr;
@@ -2290,6 +2307,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LP = std::max(L, kNQ);
     const int LF = (int)align_up(L + kFarGroup, kMarchGroup);  // the fill reads up to kFarGroup - 1 past L
     A->fstep = dalloc<FarStep>((size_t)R * LF);
+    A->fsoa = dalloc<unsigned>((size_t)R * 2 * LF);
     HIPCHK(hipMemset(A->fstep, 0, (size_t)R * LF * sizeof(FarStep)));  // steps past L: no move, no length
     A->pstep = dalloc<int4>((size_t)R1 * LP);
     A->plin = dalloc<int>((size_t)R1 * LP);
@@ -2311,11 +2329,11 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     std::copy(box, box + 6, V.box);
     V.rho_p = A->rho_p, V.table = A->table, V.support = A->support, V.W = A->W;
     V.rho_base = A->rho_alloc, V.rho_bytes = (long long)(np + 2 * margin) * (long long)sizeof(float);
-    V.fstep = A->fstep, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
+    V.fstep = A->fstep, V.fsoa = A->fsoa, V.pstep = A->pstep, V.plin = A->plin, V.kfirst = A->kfirst, V.nstep = A->nstep;
     V.nlen = A->nlen;
 
-    k_rays<<<blocks_for(R + kNearRays + kNQ, 128), 128>>>(V, A->fstep, A->pstep, A->plin, A->kfirst, A->nstep,
-                                                          A->nlen);
+    k_rays<<<blocks_for(R + kNearRays + kNQ, 128), 128>>>(V, A->fstep, A->fsoa, A->pstep, A->plin, A->kfirst,
+                                                          A->nstep, A->nlen);
     HIPCHK(hipGetLastError());
 
     build_items(A);
@@ -2614,7 +2632,7 @@ void release(prc_projector* A) {
     if (A->zdone) (void)hipEventDestroy(A->zdone), A->zdone = nullptr;
     if (A->h_stage) (void)hipHostFree(A->h_stage), A->h_stage = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
-    dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
+    dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->fsoa), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
     dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->tmaj), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
