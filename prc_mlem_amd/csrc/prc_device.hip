@@ -605,12 +605,17 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
         int len[2];
         NearStep ns[2][7];
         float rv[2][7];
+        // The pair's step records (zero past a ray's length): 16-byte scalar loads at immediate
+        // offsets from one base per pair.
+        const NearStep* np = V.nstep + r * kNearMax;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             len[h] = gload(V.nlen + r + h);
-            const NearStep* nsr = V.nstep + (r + h) * kNearMax;  // zero past the ray's length
 #pragma unroll
-            for (int s = 0; s < 7; ++s) ns[h][s] = gload(nsr + s);
+            for (int s = 0; s < 7; ++s) {
+                const v4i e = *(const CONST_AS v4i*)(np + h * kNearMax + s);
+                ns[h][s] = NearStep{__int_as_float(e[0]), e[1], e[2], e[3]};
+            }
         }
         // The whole ray stays inside the 5x5x5 block (the +inf margin around the padded density
         // keeps every address in bounds). Densities of steps 0-3 are loaded up front, those of
