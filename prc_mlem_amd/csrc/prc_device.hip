@@ -2789,7 +2789,7 @@ void batch_iteration(const Batch& B) {
 
 // Autotuner of the projector passes: every schedule of the forward and the adjoint is timed on
 // the solver's or batch's own weights and buffers (the forward into the estimate buffer, the
-// adjoint into the work buffer; the caller resets both afterwards), and the faster is kept. The
+// adjoint into the work buffer; the caller resets both afterwards), and the fastest is kept. The
 // winner depends on the batch: several volumes per launch favour the XCD-local queues, a single
 // volume the launch-order grid. PRC_TUNE=0 keeps the defaults.
 struct TuneRecord {
