@@ -832,18 +832,55 @@ __global__ __launch_bounds__(256) void k_near_fill(const Vol* __restrict__ vols,
     WT* wp = (WT*)V.wbuf + ri.off;
     WT* tb = (WT*)V.tbuf;
     const float* tpl = tmpl + (c >= 0 ? c : 0) * kNQ;
-#pragma unroll 5
-    for (int q = 0; q < kNQ; ++q) {
-        const float w = c >= 0 ? tpl[q] : 0.0f;
-        const WT h = WeightIO<WT>::store_value(w);  // exact: templates hold stored values
-        if (write) gstore(wp + (size_t)q * ri.len + lane, h);
-        const int ty = y + (q / 5 % 5 - 2), tz = z + (q / 25 - 2);
-        if (__any(w > 0.0f) && (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz) {
+    // The 125 bins by target row (dz, dy), whose records are read once: the five targets x0 - 2 ..
+    // x0 + 2 (plus the lanes) fall into segments s_lo .. s_lo + 2 of the row, and a bin's first
+    // segment is s_lo or s_lo + 1.
+#pragma unroll 1
+    for (int dzy = 0; dzy < 25; ++dzy) {
+        const int ty = y + dzy % 5 - 2, tz = z + dzy / 5 - 2;
+        const bool tin = (unsigned)ty < (unsigned)V.ny && (unsigned)tz < (unsigned)V.nz;
+        int xa_t = 0, ns = 0, s_lo = 0;
+        RowInfo e[3] = {};
+        if (tin) {
             const int t = tz * V.ny + ty;
-            const TargetPair tp = target_pair(V.tinfo + (size_t)t * nseg * R1, R1, gload(V.row_xa + t),
-                                              gload(V.row_len + t), x0 + (q % 5 - 2), q);
-            long long idx;
-            if (target_slot(tp, lane, idx) && w > 0.0f) gstore(tb + idx, h);
+            xa_t = gload(V.row_xa + t);
+            ns = (gload(V.row_len + t) + kWave - 1) / kWave;
+            const int rel = x0 - 2 - xa_t;
+            s_lo = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);  // floor
+            const RowInfo* unit0 = V.tinfo + (size_t)t * nseg * R1;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                if (s_lo + j >= 0 && s_lo + j < ns) e[j] = gload(unit0 + (size_t)(s_lo + j) * R1);
+        }
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            const int q = dzy * 5 + dx + 2;
+            const float w = c >= 0 ? tpl[q] : 0.0f;
+            const WT h = WeightIO<WT>::store_value(w);  // exact: templates hold stored values
+            if (write) gstore(wp + (size_t)q * ri.len + lane, h);
+            if (tin && __any(w > 0.0f)) {
+                // target_pair with the row's records at hand (near block: ray 0, step q).
+                const int tx0 = x0 + dx, rel = tx0 - xa_t;
+                const int s0 = rel >= 0 ? rel / kWave : -((kWave - 1 - rel) / kWave);  // s_lo or s_lo + 1
+                TargetPair tp{0, 0, kWave * (s0 + 1) - rel, 0};
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                    const int sg = s0 + hh;
+                    if (sg >= 0 && sg < ns) {
+                        const int j = sg - s_lo;  // fields picked by value: the records stay in registers
+                        const long long off = j == 0 ? e[0].off : j == 1 ? e[1].off : e[2].off;
+                        const int cnt = j == 0 ? e[0].cnt : j == 1 ? e[1].cnt : e[2].cnt;
+                        const int xa = j == 0 ? e[0].xa : j == 1 ? e[1].xa : e[2].xa;
+                        const int ln = j == 0 ? e[0].len : j == 1 ? e[1].len : e[2].len;
+                        tp.ok |= q < cnt ? 1 << hh : 0;
+                        const long long b = off - xa + (long long)q * ln + tx0;
+                        if (hh) tp.b1 = b;
+                        else tp.b0 = b;
+                    }
+                }
+                long long idx;
+                if (target_slot(tp, lane, idx) && w > 0.0f) gstore(tb + idx, h);
+            }
         }
     }
     if (c >= 0) gstore(V.W + loc, tmpl_sum[c]);
