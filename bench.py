@@ -279,7 +279,7 @@ def main():
             roofline = {"kernel": name, "bound": "valu", "achieved": None, "peak": None, "unit": None,
                         "frac": None, "traffic": None, "mean_ms": k[name]["mean_ms"]}
 
-    # Projector throughput in the survey's effective accounting: one call = the far voxel-ray
+    # Projector throughput in the effective accounting of the on-the-fly design: one call = the far voxel-ray
     # steps plus 125 near taps per active source, at 8 B (FP32) or 6 B (mixed) per step. These
     # are effective bytes, not HBM traffic (the stored-weight stream is the roofline above).
     projector = {}

@@ -1,7 +1,7 @@
 """Numerical behaviour of the drop-in `prc` executable on the GPU: the invariant suite of the
 reference's GPU tests (tests/test_solver.py:141-361), run through the CLI and raw files on
-the same seeded 16x14x12 anisotropic phantom, with the reference's tolerances (and the
-north-star's 1e-6 for the identity and conservation where it is tighter)."""
+the same seeded 16x14x12 anisotropic phantom, with the reference's tolerances (and 1e-6 for
+the identity and conservation where that is tighter)."""
 import os
 import subprocess
 from pathlib import Path

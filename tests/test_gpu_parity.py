@@ -62,7 +62,7 @@ def test_forward_adjoint_match_oracle(pair, phantom):
 
 
 def test_identity_conservation_and_support(pair, phantom):
-    # tests/test_solver.py:180-190, with the north-star bound 1e-6.
+    # tests/test_solver.py:180-190, with a 1e-6 bound.
     A, _ = pair
     _, x, y, sup = phantom
     ax, aty = A.forward(x), A.adjoint(y)
@@ -95,7 +95,7 @@ def test_table_vs_exact_range_all_isotopes(prc, phantom, iso):
 
 @pytest.mark.parametrize("beta,accel", [(0.0, False), (0.1, True), (0.2, False), (0.0, True)])
 def test_solver_per_iteration_matches_oracle(prc, pair, phantom, beta, accel):
-    # FP32 GPU vs the FP32 oracle within 1e-5 relative L2 per iteration.
+    # Bar: FP32 GPU vs the FP32 oracle within 1e-5 relative L2 per iteration.
     A, O = pair
     _, x, _, sup = phantom
     announce(f"solver beta={beta} accel={accel}")

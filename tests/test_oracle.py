@@ -10,8 +10,8 @@ from conftest import rel_l2
 from prc_mlem_amd import phantoms
 
 ISOTOPES = ["11C", "13N", "15O", "18F", "64Cu", "68Ga", "82Rb", "124I"]
-# Cutoffs quoted for the reference model (FP64 evaluation of prc.cu:65-91 in the survey of
-# the reference): 68Ga 8.487, 18F 2.115, 82Rb 16.16, 124I 9.66, 11C 3.772, 13N 4.979,
+# Cutoffs quoted for the reference model (an independent FP64 evaluation of the closed form of
+# prc.cu:65-91): 68Ga 8.487, 18F 2.115, 82Rb 16.16, 124I 9.66, 11C 3.772, 13N 4.979,
 # 15O 7.811, 64Cu 2.202 mm.
 QUOTED_CUTOFF = {"11C": 3.772, "13N": 4.979, "15O": 7.811, "18F": 2.115, "64Cu": 2.202, "68Ga": 8.487,
                  "82Rb": 16.16, "124I": 9.66}
