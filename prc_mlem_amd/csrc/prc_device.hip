@@ -142,7 +142,7 @@ struct TargetRun {      // 16 bytes
     int cnt;            // planes
     int pad;
 };
-struct FarStep {        // march table of a far ray (16 bytes: one lane loads one step)
+struct FarStep {        // march table of a far ray (16 bytes, read with wave-uniform scalar loads)
     float seg;          // path length [mm] inside the step's voxel (t_next - t, prc.cu:150)
     unsigned soff;      // 2^31 + 4 * (padded linear offset of the step's voxel from the source): the
                         // scalar offset of the step's density load (RhoReader::step)
@@ -1558,7 +1558,8 @@ __device__ inline void load_shifts(const GLOBAL_AS int* p, int (&d)[G]) {
 
 // acc[e] += sum over the segment's planes k of w[k][x] * vec0[x + shift(k)] for the lane's x =
 // c*E + e, all lanes ending with the full sums. `wseg` is the segment's first plane, `len` its
-// width (the plane stride), `list` its shifts. Zero weights (no source) are skipped by a select.
+// width (the plane stride), `list` its shifts. Zero weights (no source) are skipped by a select
+// where the gathered vector is FP32 (see below for the FP16 form).
 // The last group clamps its plane reads to the last plane and zeroes the weights past it. Each
 // group loads the next group's shifts while its own loads are in flight, so shift reads run up to
 // 2P - 1 entries past the list (into the next list or the slack: valid shifts either way).
@@ -1683,7 +1684,8 @@ __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const 
 // out = A g where g already holds f / W (prc.cu:239-254 as a gather). The target's planes are
 // streamed; for step (p, k) target t gathers g at t - D(p,k), a shift shared by the plane. The
 // input vector carries a margin wide enough for every stored shift, so no gather needs a bounds
-// check; zero weights (no source) are skipped by a select. Items are the active segments (w > 0)
+// check; zero weights (no source) are skipped by a select where the vector is FP32 (an FP16 vector
+// is finite). Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
 // SCHED selects the schedule (enum Sched). The gathered vector is FP32, or FP16 in the mixed
 // form (VT).
@@ -2350,7 +2352,6 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const int LF = (int)align_up(L + kFarGroup, kMarchGroup);  // the fill reads up to kFarGroup - 1 past L
     A->fstep = dalloc<FarStep>((size_t)R * LF);
     A->fsoa = dalloc<unsigned>((size_t)R * 2 * LF);
-    HIPCHK(hipMemset(A->fstep, 0, (size_t)R * LF * sizeof(FarStep)));  // steps past L: no move, no length
     A->pstep = dalloc<int4>((size_t)R1 * LP);
     A->plin = dalloc<int>((size_t)R1 * LP);
     HIPCHK(hipMemset(A->plin, 0, (size_t)R1 * LP * sizeof(int)));
