@@ -250,6 +250,55 @@ def test_uniform_neighbourhood_memo_matches_oracle(prc, oracle_mod, blocks):
     A.close()
 
 
+@pytest.fixture(scope="module")
+def wide_rows():
+    """150x6x5 phantom whose rows span up to three 64-voxel segments: every row has its own
+    active range (ragged last segments, segment boundaries at different x from row to row), some
+    rows hold sub-threshold gaps and one row holds no active voxel at all."""
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.2, 1.9, (5, 6, 150)).astype(np.float32)
+    for z in range(5):
+        for y in range(6):
+            a, b = int(rng.integers(0, 40)), int(rng.integers(100, 151))
+            rho[z, y, :a] = 0.01
+            rho[z, y, b:] = 0.01
+    rho[1::2, :, 70:75] = 0.02
+    rho[2, 3, :] = 0.01
+    x = rng.uniform(0, 2, rho.shape).astype(np.float32)
+    y = rng.uniform(0, 2, rho.shape).astype(np.float32)
+    return rho, x, y
+
+
+def test_multi_segment_rows_match_oracle(prc, oracle_mod, wide_rows):
+    rho, x, y = wide_rows
+    announce("multi-segment rows 150x6x5: FP32 vs oracle, mixed vs FP32")
+    A = prc.Projector(rho, SPACING, "68Ga", 32)
+    table, _ = A.table
+    rho_p = np.maximum(rho, np.float32(1e-3))
+    sup = rho_p >= np.float32(0.05)
+    assert not sup[2, 3].any() and sup.sum(axis=2).max() > 128
+    O = oracle_mod.Projector(rho_p, SPACING, "68Ga", 32, table=table)
+    w_ref = np.zeros(rho.size)
+    w_ref[O.active] = O.W
+    np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
+    ax, aty = A.forward(x), A.adjoint(y)
+    assert rel_l2(ax, O.forward(x)) <= 1e-6
+    assert rel_l2(aty, O.adjoint(y)) <= 1e-6
+    # The FP16 copies use a different lane mapping (8 entries per lane, not 4) over the same
+    # segments; they differ from FP32 by the weights' rounding only, and keep the identities.
+    M = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    mx, mty = M.forward(x), M.adjoint(y)
+    assert rel_l2(mx, ax) <= 1e-3 and rel_l2(mty, aty) <= 1e-3
+    for fx, fty in ((ax, aty), (mx, mty)):
+        lhs = np.sum(fx.astype(np.float64) * y)
+        rhs = np.sum(x.astype(np.float64) * fty)
+        assert abs(lhs / rhs - 1) <= 1e-6
+        assert abs(fx.sum(dtype=np.float64) / x[sup].sum(dtype=np.float64) - 1) <= 1e-6
+        np.testing.assert_array_equal(fx[~sup], 0)
+    A.close()
+    M.close()
+
+
 def test_concurrent_normalisation_matches_single(prc, phantom, blocks):
     # Several projectors normalised together (one stream each) give the single builds' weights.
     rho, x, y, sup = phantom
