@@ -299,6 +299,21 @@ def test_multi_segment_rows_match_oracle(prc, oracle_mod, wide_rows):
     M.close()
 
 
+def test_mixed_batch_of_wide_rows_equals_single_runs(prc, wide_rows, phantom):
+    # The batched mixed form over volumes of one and of several segments per row.
+    rho, x, _ = wide_rows
+    rho2, x2, _, _ = phantom
+    announce("mixed batch: 150x6x5 and 16x14x12 vs single runs")
+    A = prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    B = prc.Projector(rho2, SPACING, "18F", 16, precision="mixed")
+    singles = [prc.correct(A, x, 4, 0.03, 20.0, True), prc.correct(B, x2, 4, 0.1, 20.0, True)]
+    batch = prc.batch_correct([A, B], [x, x2], 4, beta=[0.03, 0.1], gamma=20.0, accelerate=True)
+    for a, b in zip(batch, singles):
+        np.testing.assert_array_equal(a, b)
+    A.close()
+    B.close()
+
+
 def test_concurrent_normalisation_matches_single(prc, phantom, blocks):
     # Several projectors normalised together (one stream each) give the single builds' weights.
     rho, x, y, sup = phantom
