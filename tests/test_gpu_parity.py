@@ -299,6 +299,22 @@ def test_multi_segment_rows_match_oracle(prc, oracle_mod, wide_rows):
     M.close()
 
 
+@pytest.mark.parametrize("rays", [2, 130])
+def test_ray_count_edges_match_oracle(prc, oracle_mod, phantom, rays):
+    # The fewest rays the CLI accepts, and more than two 64-ray chunks of the shift-list builder.
+    rho, x, y, sup = phantom
+    announce(f"ray count {rays}: GPU vs oracle 16x14x12")
+    A = prc.Projector(rho, SPACING, "68Ga", rays)
+    table, _ = A.table
+    O = oracle_mod.Projector(rho, SPACING, "68Ga", rays, table=table)
+    w_ref = np.zeros(rho.size)
+    w_ref[O.active] = O.W
+    np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
+    assert rel_l2(A.forward(x), O.forward(x)) <= 1e-6
+    assert rel_l2(A.adjoint(y), O.adjoint(y)) <= 1e-6
+    A.close()
+
+
 def test_mixed_batch_of_wide_rows_equals_single_runs(prc, wide_rows, phantom):
     # The batched mixed form over volumes of one and of several segments per row.
     rho, x, _ = wide_rows
