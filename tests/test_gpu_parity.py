@@ -299,14 +299,18 @@ def test_multi_segment_rows_match_oracle(prc, oracle_mod, wide_rows):
     M.close()
 
 
-@pytest.mark.parametrize("rays", [2, 130])
-def test_ray_count_edges_match_oracle(prc, oracle_mod, phantom, rays):
-    # The fewest rays the CLI accepts, and more than two 64-ray chunks of the shift-list builder.
+@pytest.mark.parametrize("iso,rays,spacing", [
+    ("68Ga", 2, SPACING),             # the fewest rays the CLI accepts
+    ("68Ga", 130, SPACING),           # more than two 64-ray chunks of the shift-list builder
+    ("82Rb", 32, (2.0, 2.0, 2.0)),    # isotropic voxels, the longest-range isotope
+    ("11C", 24, (0.5, 3.0, 1.0)),     # strongly anisotropic voxels, a short-range isotope
+])
+def test_geometry_edges_match_oracle(prc, oracle_mod, phantom, iso, rays, spacing):
     rho, x, y, sup = phantom
-    announce(f"ray count {rays}: GPU vs oracle 16x14x12")
-    A = prc.Projector(rho, SPACING, "68Ga", rays)
+    announce(f"{iso} rays={rays} spacing={spacing}: GPU vs oracle 16x14x12")
+    A = prc.Projector(rho, spacing, iso, rays)
     table, _ = A.table
-    O = oracle_mod.Projector(rho, SPACING, "68Ga", rays, table=table)
+    O = oracle_mod.Projector(rho, spacing, iso, rays, table=table)
     w_ref = np.zeros(rho.size)
     w_ref[O.active] = O.W
     np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
