@@ -299,18 +299,20 @@ def test_multi_segment_rows_match_oracle(prc, oracle_mod, wide_rows):
     M.close()
 
 
-@pytest.mark.parametrize("iso,rays,spacing", [
-    ("68Ga", 2, SPACING),             # the fewest rays the CLI accepts
-    ("68Ga", 130, SPACING),           # more than two 64-ray chunks of the shift-list builder
-    ("82Rb", 32, (2.0, 2.0, 2.0)),    # isotropic voxels, the longest-range isotope
-    ("11C", 24, (0.5, 3.0, 1.0)),     # strongly anisotropic voxels, a short-range isotope
+@pytest.mark.parametrize("iso,rays,spacing,air_cap,support", [
+    ("68Ga", 2, SPACING, 50.0, 0.05),           # the fewest rays the CLI accepts
+    ("68Ga", 130, SPACING, 50.0, 0.05),         # more than two 64-ray chunks of the shift-list builder
+    ("82Rb", 32, (2.0, 2.0, 2.0), 50.0, 0.05),  # isotropic voxels, the longest-range isotope
+    ("11C", 24, (0.5, 3.0, 1.0), 50.0, 0.05),   # strongly anisotropic voxels, a short-range isotope
+    ("82Rb", 32, SPACING, 3.0, 0.05),           # rays die after about two voxels of air
+    ("68Ga", 32, SPACING, 10.0, 0.5),           # a higher support threshold: less tissue, more air
 ])
-def test_geometry_edges_match_oracle(prc, oracle_mod, phantom, iso, rays, spacing):
-    rho, x, y, sup = phantom
-    announce(f"{iso} rays={rays} spacing={spacing}: GPU vs oracle 16x14x12")
-    A = prc.Projector(rho, spacing, iso, rays)
+def test_geometry_edges_match_oracle(prc, oracle_mod, phantom, iso, rays, spacing, air_cap, support):
+    rho, x, y, _ = phantom
+    announce(f"{iso} rays={rays} spacing={spacing} air_cap={air_cap} support={support}: GPU vs oracle 16x14x12")
+    A = prc.Projector(rho, spacing, iso, rays, support_rho=support, air_cap_mm=air_cap)
     table, _ = A.table
-    O = oracle_mod.Projector(rho, spacing, iso, rays, table=table)
+    O = oracle_mod.Projector(rho, spacing, iso, rays, support_rho=support, air_cap=air_cap, table=table)
     w_ref = np.zeros(rho.size)
     w_ref[O.active] = O.W
     np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
