@@ -316,9 +316,18 @@ def test_geometry_edges_match_oracle(prc, oracle_mod, phantom, iso, rays, spacin
     w_ref = np.zeros(rho.size)
     w_ref[O.active] = O.W
     np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=2e-6, atol=0)
-    assert rel_l2(A.forward(x), O.forward(x)) <= 1e-6
-    assert rel_l2(A.adjoint(y), O.adjoint(y)) <= 1e-6
+    ax, aty = A.forward(x), A.adjoint(y)
+    assert rel_l2(ax, O.forward(x)) <= 1e-6
+    assert rel_l2(aty, O.adjoint(y)) <= 1e-6
+    # The mixed form on the same geometry: FP16 weight rounding only, identities kept.
+    M = prc.Projector(rho, spacing, iso, rays, support_rho=support, air_cap_mm=air_cap, precision="mixed")
+    mx, mty = M.forward(x), M.adjoint(y)
+    assert rel_l2(mx, ax) <= 1e-3 and rel_l2(mty, aty) <= 1e-3
+    lhs = np.sum(mx.astype(np.float64) * y)
+    rhs = np.sum(x.astype(np.float64) * mty)
+    assert abs(lhs / rhs - 1) <= 1e-6
     A.close()
+    M.close()
 
 
 def test_mixed_batch_of_wide_rows_equals_single_runs(prc, wide_rows, phantom):
