@@ -330,6 +330,23 @@ def test_geometry_edges_match_oracle(prc, oracle_mod, phantom, iso, rays, spacin
     M.close()
 
 
+@pytest.mark.parametrize("iso", ["68Ga", "124I"])
+def test_exact_range_matches_oracle(prc, oracle_mod, phantom, iso):
+    # --exact-range: the survival model evaluated at every step (prc.cu:75, 625) on both sides.
+    # Device and host transcendentals may differ by an ulp, and a deposit is a difference of two
+    # survivals: the bar is the per-iteration one, 1e-5.
+    rho, x, y, _ = phantom
+    announce(f"exact range {iso}: GPU vs oracle 16x14x12")
+    A = prc.Projector(rho, SPACING, iso, 32, exact_range=True)
+    O = oracle_mod.Projector(rho, SPACING, iso, 32, exact=True)
+    w_ref = np.zeros(rho.size)
+    w_ref[O.active] = O.W
+    np.testing.assert_allclose(A.W.ravel(), w_ref, rtol=1e-5, atol=0)
+    ef, ea = rel_l2(A.forward(x), O.forward(x)), rel_l2(A.adjoint(y), O.adjoint(y))
+    assert ef <= 1e-5 and ea <= 1e-5, (ef, ea)
+    A.close()
+
+
 def test_mixed_batch_of_wide_rows_equals_single_runs(prc, wide_rows, phantom):
     # The batched mixed form over volumes of one and of several segments per row.
     rho, x, _ = wide_rows
