@@ -70,6 +70,52 @@ def test_ray_set_invariants(oracle_mod):
         np.testing.assert_allclose(d32, dirs, atol=3e-7)
 
 
+def golden_multipliers():
+    """The ray set's four 32-bit multipliers from their closed forms (prc.cu:115-128): the golden
+    ratio's round(2^32 / phi) for the Fibonacci azimuth, and the R3 sequence's round(2^32 / p^d),
+    d = 1, 2, 3, where p is the real root > 1 of p^4 = p + 1."""
+    phi = (1.0 + math.sqrt(5.0)) / 2.0
+    lo, hi = 1.0, 2.0
+    for _ in range(200):  # bisection for p^4 - p - 1 = 0
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if mid ** 4 - mid - 1.0 < 0 else (lo, mid)
+    p = 0.5 * (lo + hi)
+    return round(2 ** 32 / phi), [round(2 ** 32 / p ** d) for d in (1, 2, 3)]
+
+
+def closed_form_rays(ks, n):
+    """Start offsets and directions of rays ks of an n-ray set in FP64 from the closed forms, with
+    the products wrapped modulo 2^32 as unsigned integers."""
+    kphi, kr3 = golden_multipliers()
+    ks = np.asarray(ks, np.int64)
+    m = ks >> 1
+    sign = np.where(ks & 1, -1.0, 1.0)
+    z = 1.0 - (m + 0.5) / ((n + 1) >> 1)
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    az = 2.0 * np.pi * ((m * kphi) % 2 ** 32) / 2.0 ** 32
+    dirs = sign[:, None] * np.stack([r * np.cos(az), r * np.sin(az), z], axis=1)
+    starts = sign[:, None] * np.stack([((2 ** 31 + m * kd) % 2 ** 32) / 2.0 ** 32 - 0.5 for kd in kr3], axis=1)
+    return starts, dirs
+
+
+def test_ray_multipliers_are_the_closed_forms(oracle_mod):
+    """A one-digit change of any multiplier moves the azimuth or the offsets of ray pair m by
+    m * change / 2^32 turns, far above the 1e-12 bar at the pair indices sampled here."""
+    kphi, kr3 = golden_multipliers()
+    # The integers the closed forms give (fractional parts 0.497, 0.819, 0.983, 0.185: no tie).
+    assert (kphi, *kr3) == (2654435769, 3518319155, 2882110345, 2360945575)
+    rng = np.random.default_rng(9)
+    for n in (2, 64, 2048, 1 << 21):
+        ks = np.unique(np.concatenate([np.arange(min(n, 8)), rng.integers(0, n, 400)]))
+        want_s, want_d = closed_form_rays(ks, n)
+        got = [oracle_mod.ray_f64(int(k), n) for k in ks]
+        np.testing.assert_allclose(np.array([g[0] for g in got]), want_s, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(np.array([g[1] for g in got]), want_d, rtol=0, atol=1e-12)
+        got32 = [oracle_mod.ray_f32(int(k), n) for k in ks]
+        np.testing.assert_allclose(np.array([g[0] for g in got32]), want_s, rtol=0, atol=1e-7)
+        np.testing.assert_allclose(np.array([g[1] for g in got32]), want_d, rtol=0, atol=4e-7)
+
+
 def brute_force_walk(start, d, spacing, v, shape, n_max=400):
     """Voxels crossed by the ray v + start + t d (t in mm) and the path length in each, by
     exact FP64 plane intersections (no incremental DDA)."""
@@ -284,6 +330,54 @@ def test_rdp_update_matches_finite_differences_and_minimisation(oracle_mod):
         opt = minimize_scalar(objective, bounds=(1e-12, 10 * (em + fj) + 10), method="bounded",
                               options={"xatol": 1e-12})
         assert got == pytest.approx(opt.x, rel=2e-4, abs=1e-7)
+
+
+def test_rdp_distance_weights_on_anisotropic_neighbourhood(oracle_mod):
+    """Centre voxel of a 3x3x3 block at spacing (0.7, 1.3, 2.1) mm with all 26 neighbours (6 face,
+    12 edge, 8 corner) in the support: the update must be the minimiser of the surrogate whose
+    gradient and curvature come, by finite differences, from the penalty
+    sum_k w_k phi(f_j, f_k) with w_k = min(s) / |(dx sx, dy sy, dz sz)| (prc.cu:288-301). The
+    weights are computed here from the geometry alone; min(s) / |d|^2, max(s) / |d| or unit weights
+    give a different minimiser at every one of these cases."""
+    L = oracle_mod.lib()
+    sp = (0.7, 1.3, 2.1)
+    offs = [(dx, dy, dz) for dz in (-1, 0, 1) for dy in (-1, 0, 1) for dx in (-1, 0, 1) if (dx, dy, dz) != (0, 0, 0)]
+    wts = {o: min(sp) / math.sqrt((o[0] * sp[0]) ** 2 + (o[1] * sp[1]) ** 2 + (o[2] * sp[2]) ** 2) for o in offs}
+    rng = np.random.default_rng(7)
+    centre = 13  # (1, 1, 1) of the 3x3x3 block, X fastest
+    for case in range(40):
+        f = np.exp(rng.uniform(-1.5, 1.5, 27))
+        back = np.exp(rng.uniform(-0.5, 0.5, 27))
+        sup = np.ones(27, np.uint8)
+        if case % 4 == 3:  # some neighbours outside the support drop out of the sum
+            sup[rng.choice([i for i in range(27) if i != centre], 5, replace=False)] = 0
+        beta, gamma, eps = float(np.exp(rng.uniform(-3, -0.5))), float(rng.choice([0.0, 2.0, 20.0])), 0.05
+        fj = f[centre]
+        nbrs = [(wts[o], f[centre + o[0] + 3 * o[1] + 9 * o[2]]) for o in offs
+                if sup[centre + o[0] + 3 * o[1] + 9 * o[2]]]
+
+        def total(x):
+            return sum(w * penalty(x, fk, gamma, eps) for w, fk in nbrs)
+
+        h = 1e-6 * fj
+        G = (total(fj + h) - total(fj - h)) / (2 * h)
+        C = 0.0
+        for w, fk in nbrs:  # 2 w max(half_quad, FD second derivative), neighbour by neighbour
+            def g1(x, fk=fk):
+                return (penalty(x + h, fk, gamma, eps) - penalty(x - h, fk, gamma, eps)) / (2 * h)
+            hess = (g1(fj * (1 + 1e-4)) - g1(fj * (1 - 1e-4))) / (2e-4 * fj)
+            D = fj + fk + gamma * abs(fj - fk) + eps
+            C += 2 * w * max((2 * (fj + fk + eps) + gamma * abs(fj - fk)) / D ** 2, hess)
+        em = fj * back[centre]
+
+        def objective(x):
+            dx = x - fj
+            return -em * math.log(x) + x + beta * (G * dx + C * dx * dx / 2)
+
+        opt = minimize_scalar(objective, bounds=(1e-12, 10 * (em + fj) + 10), method="bounded",
+                              options={"xatol": 1e-13})
+        got = L.orc_rdp_update_f64(3, 3, 3, *sp, f, back, sup, centre, beta, gamma, eps)
+        assert got == pytest.approx(opt.x, rel=1e-4, abs=1e-9), case
 
 
 def test_oracle_solver_invariants(oracle_mod, small):
