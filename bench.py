@@ -182,9 +182,9 @@ def main():
 
     # Mixed + batched: FP16 weights, every stage one launch over the batch, graph-captured loop.
     # Sequential: normalise the batch's projectors, then iterate. Pipelined (the headline form):
-    # two batches in flight, so that the normalisation of one batch (VALU-bound march, on each
-    # projector's own stream) overlaps the iterations of the other (HBM-bound passes, on the
-    # batch's stream). Every step still normalises and corrects `batch` volumes.
+    # two batches in flight, each building and iterating on its own stream, so that the
+    # normalisation of one batch (VALU-bound march) overlaps the iterations of the other
+    # (HBM-bound passes). Every step still normalises and corrects `batch` volumes.
     mixed = None
     if args.only in ("both", "mixed"):
         sets = 2 if args.pipeline else 1
@@ -195,7 +195,7 @@ def main():
         turn = [0]
 
         def step_set(k):
-            prc.normalise_all(As[k])  # the batch's projectors build concurrently, one stream each
+            Bs[k].normalise()  # the batch's projectors build on the batch's stream, then it iterates
             Bs[k].run(ITERS)
 
         def step_seq():

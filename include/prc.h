@@ -165,6 +165,12 @@ int prc_solver_sums(prc_solver* solver, double* estimate_sum, double* support_su
  * Volumes must share the precision and the acceleration flag; beta and gamma may differ. */
 int prc_batch_create(int n, prc_projector* const* projectors, const float* const* host_g,
                      const prc_solve_params* params, prc_batch** out);
+/* Normalise every projector of the batch again (same result as prc_projector_normalise on
+ * each), on the batch's own stream: it returns once the builds' buffer sizes are known and
+ * leaves the rest queued, so a prc_batch_run that follows sees the new weights while the
+ * iterations of other batches keep running beside the build. No counterpart in the reference
+ * (Projector::Projector, prc.cu:405-422, builds one volume on the default stream). */
+int prc_batch_normalise(prc_batch* batch);
 /* Restart every volume from its data and run `iterations` (asynchronous). The batch reads
  * the projectors' current weights, so a projector may be normalised again between runs. */
 int prc_batch_run(prc_batch* batch, int iterations);

@@ -87,6 +87,7 @@ SIGNATURES = {
     "prc_solver_sums": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "prc_batch_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                    C.POINTER(SolveParams), C.POINTER(C.c_void_p)]),
+    "prc_batch_normalise": (C.c_int, [C.c_void_p]),
     "prc_batch_run": (C.c_int, [C.c_void_p, C.c_int]),
     "prc_batch_sync": (C.c_int, [C.c_void_p]),
     "prc_batch_estimate": (C.c_int, [C.c_void_p, C.c_int, _F32P]),
@@ -304,6 +305,11 @@ class Batch:
             self.handle = None
 
     __del__ = close
+
+    def normalise(self):
+        """Rebuild every projector's weights on the batch's stream (queued before the next run)."""
+        _check(lib().prc_batch_normalise(self.handle))
+        return self
 
     def run(self, iterations):
         _check(lib().prc_batch_run(self.handle, int(iterations)))

@@ -2107,16 +2107,9 @@ struct prc_projector {
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
     int* queue = nullptr;       // work-queue counters of the projector passes
-    // The normalisation runs on a stream of its own that does not wait for other streams, so
-    // that it can overlap a batch that is iterating on other projectors. It first waits for the
-    // null stream (solvers) and for the streams of the batches that use this projector.
-    hipStream_t stream = nullptr;
-    // The target-major copy must be zero before the near block and the fill write its deposits.
-    // When its buffer is kept from the previous normalisation, a second stream clears it while
-    // the count pass and the layout run (zeroed records that the clear is queued there).
-    hipStream_t zstream = nullptr;
-    hipEvent_t zdone = nullptr;
-    bool zeroed = false;
+    // A normalisation runs on the library's build stream, or on the stream of the batch that asks
+    // for it (prc_batch_normalise). It first waits for the null stream (solvers) and for the
+    // streams of the other batches that use this projector.
     std::vector<hipStream_t> users;
     // Pinned staging of the sizes normalise_begin reads back (so the read-back does not block
     // the host): total[4] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
@@ -2219,10 +2212,9 @@ void build_items(prc_projector* A) {
 }
 
 template <class WT>
-void launch_far(prc_projector* A, int pass) {
+void launch_far(prc_projector* A, int pass, hipStream_t st) {
     const bool exact = A->hv.exact, count = !A->counted;
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
-    hipStream_t st = A->stream;
     if (pass == 1) {
         ProfScope ps("far_count", st);
         if (count)
@@ -2243,27 +2235,27 @@ void launch_far(prc_projector* A, int pass) {
 }
 
 template <class WT>
-void launch_near(prc_projector* A) {
+void launch_near(prc_projector* A, hipStream_t st) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
-    ProfScope ps("near", A->stream);
+    ProfScope ps("near", st);
     // The voxel visits are counted by a projector's first build only.
     const bool count = !A->counted;
     unsigned long long* vrs = A->counters;
     auto near_list = [&](auto exact_c, auto count_c) {
         constexpr bool E = decltype(exact_c)::value, C = decltype(count_c)::value;
-        k_near_list<E, WT, C><<<blocks_for(A->n_list, 256), 256, 0, A->stream>>>(A->dv, A->near_list, A->n_list,
+        k_near_list<E, WT, C><<<blocks_for(A->n_list, 256), 256, 0, st>>>(A->dv, A->near_list, A->n_list,
                                                                                 A->tmpl, A->tmpl_sum, vrs);
     };
     auto near_rows = [&](auto exact_c, auto count_c) {
         constexpr bool E = decltype(exact_c)::value, C = decltype(count_c)::value;
-        k_near<E, WT, C><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, vrs);
+        k_near<E, WT, C><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, vrs);
     };
     using T = std::true_type;
     using F = std::false_type;
     if (A->memo) {  // march the listed sources, then copy the templates to the uniform ones
         if (A->hv.exact) count ? near_list(T(), T()) : near_list(T(), F());
         else count ? near_list(F(), T()) : near_list(F(), F());
-        k_near_fill<WT><<<nb, 256, 0, A->stream>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
+        k_near_fill<WT><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, A->cls, A->tmpl, A->tmpl_sum);
     } else if (A->hv.exact) {
         count ? near_rows(T(), T()) : near_rows(T(), F());
     } else {
@@ -2417,14 +2409,14 @@ void ensure_weights(void*& raw, void*& buf, long long& cap, long long entries, i
 }
 
 // row_base / list_base = exclusive prefix sums of row_tot / unit_planes over the segments.
-void scan_units(prc_projector* A, long long* total_a, long long* total_b) {
+void scan_units(prc_projector* A, long long* total_a, long long* total_b, hipStream_t st) {
     const int units = A->hv.ny * A->hv.nz * A->hv.nseg;
     const int tiles = (int)blocks_for(units, kScanTile);
     long long* ta = A->tile_sums;
     long long* tb = A->tile_sums + tiles;
-    k_tile_sums<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb);
-    k_scan_tiles<<<1, kScanTile, 0, A->stream>>>(ta, tb, tiles, total_a, total_b);
-    k_scan_in_tiles<<<tiles, kScanTile, 0, A->stream>>>(A->row_tot, A->unit_planes, units, ta, tb, A->row_base,
+    k_tile_sums<<<tiles, kScanTile, 0, st>>>(A->row_tot, A->unit_planes, units, ta, tb);
+    k_scan_tiles<<<1, kScanTile, 0, st>>>(ta, tb, tiles, total_a, total_b);
+    k_scan_in_tiles<<<tiles, kScanTile, 0, st>>>(A->row_tot, A->unit_planes, units, ta, tb, A->row_base,
                                                         A->list_base);
 }
 
@@ -2445,19 +2437,13 @@ void ensure_list(int*& list, long long& cap, long long entries) {
 // for the forward) and W. One host synchronisation (the buffer sizes); buffers are reused.
 // Phase 1, enqueued on the projector's stream: counts, layout, and the read-back of the sizes into
 // the projector's pinned staging (normalise_finish reads them after a stream sync).
-void normalise_begin(prc_projector* A) {
+void normalise_begin(prc_projector* A, hipStream_t st) {
     Vol& V = A->hv;
     const size_t units = (size_t)V.ny * V.nz * V.nseg;
     const int R1 = V.n_rays + 1;
-    hipStream_t st = A->stream;
     HIPCHK(hipStreamSynchronize(nullptr));  // solver work on this projector
-    for (hipStream_t u : A->users) HIPCHK(hipStreamSynchronize(u));  // batches on this projector
-    A->zeroed = false;
-    if (A->traw) {  // nothing reads the target-major copy now: clear it beside the count pass
-        HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, A->zstream));
-        HIPCHK(hipEventRecord(A->zdone, A->zstream));
-        A->zeroed = true;
-    }
+    for (hipStream_t u : A->users)          // other batches on this projector (st itself is in order)
+        if (u != st) HIPCHK(hipStreamSynchronize(u));
     HIPCHK(hipMemsetAsync(A->rinfo, 0, units * R1 * sizeof(RowInfo), st));
     HIPCHK(hipMemsetAsync(A->counters, 0, 4 * sizeof(unsigned long long), st));
     HIPCHK(hipMemsetAsync(A->W, 0, A->n * sizeof(float), st));
@@ -2473,18 +2459,18 @@ void normalise_begin(prc_projector* A) {
             (int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
         HIPCHK(hipGetLastError());
     }
-    if (A->wbytes == 2) launch_far<__half>(A, 1);
-    else launch_far<float>(A, 1);
+    if (A->wbytes == 2) launch_far<__half>(A, 1, st);
+    else launch_far<float>(A, 1, st);
     {
         ProfScope ps("layout", st);
         const unsigned nu = blocks_for((long long)units, 256), nr = blocks_for((long long)units * R1, 256);
         k_row_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
-        scan_units(A, A->total, A->total + 2);
+        scan_units(A, A->total, A->total + 2, st);
         k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
         k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
         k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t);
         k_target_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
-        scan_units(A, A->total + 1, A->total + 3);
+        scan_units(A, A->total + 1, A->total + 3, st);
         k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns, A->tmaj);
         HIPCHK(hipGetLastError());
     }
@@ -2498,9 +2484,8 @@ void normalise_begin(prc_projector* A) {
 
 // Phase 2, after phase 1 has completed: buffers, then the near block and the fill, enqueued on the
 // projector's stream.
-void normalise_finish(prc_projector* A) {
+void normalise_finish(prc_projector* A, hipStream_t st) {
     Vol& V = A->hv;
-    hipStream_t st = A->stream;
     const int R1 = V.n_rays + 1;
     const long long* total = (const long long*)A->h_stage;
     const int* ctr = (const int*)(total + 4);
@@ -2518,10 +2503,6 @@ void normalise_finish(prc_projector* A) {
     if (A->margin < 0) A->margin = margin;
     else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
     ensure_weights(A->wraw, A->wbuf, A->wcap, total[0], A->wbytes);
-    if (A->zeroed && total[1] > A->tcap) {  // the buffer grows: the cleared one is dropped
-        HIPCHK(hipEventSynchronize(A->zdone));
-        A->zeroed = false;
-    }
     ensure_weights(A->traw, A->tbuf, A->tcap, total[1], A->wbytes);
     if (total[2] > INT_MAX - kListSlack || total[3] > INT_MAX - kListSlack)
         fail(PRC_ERR_INVALID, "the system matrix holds too many planes for 32-bit shift-list offsets");
@@ -2534,8 +2515,8 @@ void normalise_finish(prc_projector* A) {
     A->stats.target_planes = total[3];
     V.wbuf = A->wbuf, V.tbuf = A->tbuf, V.slist = A->slist, V.tlist = A->tlist;
     HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
-    if (A->zeroed) HIPCHK(hipStreamWaitEvent(st, A->zdone, 0));
-    else HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
+    // The fill writes only the target-major entries that receive a deposit: the rest must be zero.
+    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
     {
         const size_t units = (size_t)V.ny * V.nz * V.nseg;
         const unsigned nb = blocks_for((long long)units, 256 / kWave);
@@ -2544,30 +2525,44 @@ void normalise_finish(prc_projector* A) {
         HIPCHK(hipGetLastError());
     }
     if (A->wbytes == 2) {
-        launch_near<__half>(A);
-        launch_far<__half>(A, 2);
+        launch_near<__half>(A, st);
+        launch_far<__half>(A, 2, st);
     } else {
-        launch_near<float>(A);
-        launch_far<float>(A, 2);
+        launch_near<float>(A, st);
+        launch_far<float>(A, 2, st);
     }
 }
 
-// The normalisation pass (Projector::Projector, prc.cu:405-422) of several projectors at once:
-// each runs on its own stream, so their kernels fill each other's tails. Returns when all are done.
+// The library's build stream: normalisations that no batch asks for run here, one after another
+// (each march kernel fills the device by itself). Created on first use; the library keeps few
+// streams so that a batch's stream gets a hardware queue of its own and its work overlaps the
+// builds instead of queueing behind them.
+hipStream_t g_build_stream = nullptr;
+hipStream_t build_stream() {
+    if (!g_build_stream) HIPCHK(hipStreamCreateWithFlags(&g_build_stream, hipStreamNonBlocking));
+    return g_build_stream;
+}
+
+// The normalisation pass (Projector::Projector, prc.cu:405-422) of several projectors, enqueued on
+// stream st: every projector's count and layout, one host synchronisation for the buffer sizes,
+// then every projector's near block and fill. With `wait` the call returns when the builds are
+// done; otherwise they stay queued on st (a batch's iterations queued after them see the result).
 void read_counters(prc_projector* A);
 
-void normalise_projectors(prc_projector* const* As, int n) {
-    for (int i = 0; i < n; ++i) normalise_begin(As[i]);
-    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
-    for (int i = 0; i < n; ++i) normalise_finish(As[i]);
-    for (int i = 0; i < n; ++i) HIPCHK(hipStreamSynchronize(As[i]->stream));
+void normalise_projectors(prc_projector* const* As, int n, hipStream_t st, bool wait) {
+    for (int i = 0; i < n; ++i) normalise_begin(As[i], st);
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < n; ++i) normalise_finish(As[i], st);
     // The march statistics depend only on the density and the geometry, which a projector keeps:
     // they are counted by its first normalisation and kept.
+    bool first = false;
+    for (int i = 0; i < n; ++i) first = first || !As[i]->counted;
+    if (wait || first) HIPCHK(hipStreamSynchronize(st));
     for (int i = 0; i < n; ++i)
         if (!As[i]->counted) read_counters(As[i]), As[i]->counted = true;
 }
 
-void normalise_projector(prc_projector* A) { normalise_projectors(&A, 1); }
+void normalise_projector(prc_projector* A) { normalise_projectors(&A, 1, build_stream(), true); }
 
 // Host replay of the layout and of every index range the forward and adjoint read (PRC_AUDIT=1):
 // the runs of each segment must follow one another from its 64-entry-aligned start, its plane
@@ -2670,9 +2665,8 @@ void read_counters(prc_projector* A) {
 }
 
 void release(prc_projector* A) {
-    if (A->stream) (void)hipStreamSynchronize(A->stream), (void)hipStreamDestroy(A->stream), A->stream = nullptr;
-    if (A->zstream) (void)hipStreamSynchronize(A->zstream), (void)hipStreamDestroy(A->zstream), A->zstream = nullptr;
-    if (A->zdone) (void)hipEventDestroy(A->zdone), A->zdone = nullptr;
+    if (g_build_stream) (void)hipStreamSynchronize(g_build_stream);
+    for (hipStream_t u : A->users) (void)hipStreamSynchronize(u);
     if (A->h_stage) (void)hipHostFree(A->h_stage), A->h_stage = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->fsoa), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
@@ -2948,6 +2942,12 @@ struct prc_batch {
     int exec_iterations = -1;
     float* final_f = nullptr;
     Vol* dv = nullptr;
+    // Pinned staging of the descriptors and starting scalars that every run uploads, so that the
+    // uploads are queued on the batch's stream without waiting for it; `staged` marks when the
+    // last upload has read them.
+    Vol* h_vols = nullptr;
+    Scal* h_sc = nullptr;
+    hipEvent_t staged = nullptr;
 };
 
 namespace {
@@ -2961,6 +2961,10 @@ void batch_release(prc_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->B.stream) (void)hipStreamDestroy(b->B.stream);
+    if (b->staged) (void)hipEventDestroy(b->staged);
+    if (b->h_vols) (void)hipHostFree(b->h_vols);
+    if (b->h_sc) (void)hipHostFree(b->h_sc);
+    b->staged = nullptr, b->h_vols = nullptr, b->h_sc = nullptr;
     for (void* p : b->owned) (void)hipFree(p);
     b->owned.clear();
 }
@@ -2974,11 +2978,15 @@ void batch_reset(prc_batch* b) {
         b->vols[v] = b->A[v]->hv;
         b->vols[v].vbase = vb;
     }
-    HIPCHK(hipMemcpyAsync(b->dv, b->vols.data(), b->n * sizeof(Vol), hipMemcpyHostToDevice, B.stream));
+    HIPCHK(hipEventSynchronize(b->staged));  // the previous upload has read the staging
+    std::copy(b->vols.begin(), b->vols.end(), b->h_vols);
+    std::copy(b->sc0.begin(), b->sc0.end(), b->h_sc);
+    HIPCHK(hipMemcpyAsync(b->dv, b->h_vols, b->n * sizeof(Vol), hipMemcpyHostToDevice, B.stream));
     B.f = b->base[0], B.prev = b->base[1], B.y = b->base[2], B.hprev = b->base[3], B.est = b->base[4];
     HIPCHK(hipMemcpyAsync(B.f, B.g, b->total * sizeof(float), hipMemcpyDeviceToDevice, B.stream));
     for (int i = 1; i < 5; ++i) HIPCHK(hipMemsetAsync(b->base[i], 0, b->total * sizeof(float), B.stream));
-    HIPCHK(hipMemcpyAsync(B.sc, b->sc0.data(), b->n * sizeof(Scal), hipMemcpyHostToDevice, B.stream));
+    HIPCHK(hipMemcpyAsync(B.sc, b->h_sc, b->n * sizeof(Scal), hipMemcpyHostToDevice, B.stream));
+    HIPCHK(hipEventRecord(b->staged, B.stream));
 }
 
 void batch_run(prc_batch* b, int iterations) {
@@ -3058,9 +3066,6 @@ int prc_projector_create(const prc_geometry* geometry, const float* host_rho, co
         A->g = *geometry;
         try {
             const double t0 = now_s();
-            HIPCHK(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
-            HIPCHK(hipStreamCreateWithFlags(&A->zstream, hipStreamNonBlocking));
-            HIPCHK(hipEventCreateWithFlags(&A->zdone, hipEventDisableTiming));
             setup_projector(A, host_rho, host_table);
             HIPCHK(hipDeviceSynchronize());  // setup ran on the null stream
             normalise_projector(A);
@@ -3092,7 +3097,7 @@ int prc_projectors_normalise(int n, prc_projector* const* projectors) {
             for (int j = 0; j < i; ++j)
                 if (projectors[j] == projectors[i]) fail(PRC_ERR_INVALID, "a projector is listed twice");
         }
-        normalise_projectors(projectors, n);
+        normalise_projectors(projectors, n, build_stream(), true);
     });
 }
 
@@ -3397,6 +3402,10 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             B.sc = (Scal*)own(dalloc<Scal>(n));
             B.partial = (double*)own(dalloc<double>(2 * all.size()));
             HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+            HIPCHK(hipEventRecord(b->staged, B.stream));
+            HIPCHK(hipHostMalloc((void**)&b->h_vols, n * sizeof(Vol)));
+            HIPCHK(hipHostMalloc((void**)&b->h_sc, n * sizeof(Scal)));
             for (int v = 0; v < n; ++v) projectors[v]->users.push_back(B.stream);
             batch_reset(b);
             tune_passes(B);
@@ -3407,6 +3416,13 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             throw;
         }
         *out = b;
+    });
+}
+
+int prc_batch_normalise(prc_batch* b) {
+    return guarded([&] {
+        if (!b) fail(PRC_ERR_INVALID, "null argument");
+        normalise_projectors(b->A.data(), b->n, b->B.stream, false);
     });
 }
 
