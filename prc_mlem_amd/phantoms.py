@@ -7,6 +7,11 @@
 * thorax_phantom: 128x128x159 at 0.776x0.776x0.795 mm (68Ga workload): water elliptic body
   (45x35 mm semi-axes), two lungs at 0.30, a spine cylinder at 1.85, eight hot spheres,
   some placed on lung and bone boundaries.
+* random_volume: one volume of the uneven batch (grid 96..160 x 96..160 x 80..200, voxels
+  0.5-1.2 mm, 18F/11C/64Cu/68Ga, 1e6-1e8 counts, beta in [0, 0.2]): the cylinder generator
+  with random ellipsoid lungs, bone inserts at 1.6-1.9 and 2-10 hot spheres.
+* clinical_thorax: 344x344x127 at 2.04x2.04x2.03 mm: water body, lungs 0.26-0.32, ribs and
+  spine at 1.85, a heart wall at 20x background and 10 lesions of radius 3-12 mm at 3-8:1.
 Counts are applied by `poisson_counts` after the caller blurs the truth with the forward
 projector (the reference's own --mode forward).
 """
@@ -85,3 +90,83 @@ def poisson_counts(blurred, total, seed):
     b = np.asarray(blurred, np.float64)
     lam = b * (total / b.sum())
     return np.random.default_rng(seed).poisson(lam).astype(np.float32)
+
+
+def _ellipsoid(z, y, x, centre, semi):
+    cz, cy, cx = centre
+    az, ay, ax = semi
+    return ((x - cx) / ax) ** 2 + ((y - cy) / ay) ** 2 + ((z - cz) / az) ** 2 < 1.0
+
+
+def random_volume(rng):
+    """One volume of the uneven batch. Returns a dict with shape (NZ, NY, NX), spacing
+    (DX, DY, DZ), isotope, counts, beta, rho and truth."""
+    nx, ny = (int(v) for v in rng.integers(96, 161, 2))
+    nz = int(rng.integers(80, 201))
+    sxy, sz = float(rng.uniform(0.5, 1.2)), float(rng.uniform(0.5, 1.2))
+    shape, spacing = (nz, ny, nx), (sxy, sxy, sz)
+    z, y, x = _grid(shape, spacing)
+    half = 0.5 * min(nx, ny) * sxy
+    radius = rng.uniform(0.7, 0.9) * half
+    body = x ** 2 + y ** 2 < radius ** 2
+    rho = np.where(body, np.float32(1.0), AIR).astype(np.float32)
+    act = body.astype(np.float32)
+    zext = 0.5 * nz * sz
+    for side in (-1.0, 1.0):  # two random ellipsoid lungs
+        semi = (rng.uniform(0.3, 0.6) * zext, rng.uniform(0.3, 0.5) * radius, rng.uniform(0.2, 0.3) * radius)
+        centre = (rng.uniform(-0.2, 0.2) * zext, rng.uniform(-0.2, 0.1) * radius, side * rng.uniform(0.35, 0.5) * radius)
+        lung = _ellipsoid(z, y, x, centre, semi) & body
+        rho[lung] = np.float32(rng.uniform(0.25, 0.35))
+        act[lung] = 0.3
+    for _ in range(int(rng.integers(1, 4))):  # bone inserts: Z-cylinders at 1.6-1.9
+        r = rng.uniform(0.05, 0.12) * radius
+        cx, cy = rng.uniform(-0.5, 0.5, 2) * radius
+        bone = ((x - cx) ** 2 + (y - cy) ** 2 < r * r) & body
+        rho[bone] = np.float32(rng.uniform(1.6, 1.9))
+    for _ in range(int(rng.integers(2, 11))):  # hot spheres of radius 1.5-6 mm at 2-8:1
+        r = rng.uniform(1.5, 6.0)
+        cz, cy, cx = rng.uniform(-0.8, 0.8) * zext, *(rng.uniform(-0.7, 0.7, 2) * radius)
+        act[((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < r * r) & body] = np.float32(rng.uniform(2.0, 8.0))
+    return {"shape": shape, "spacing": spacing, "isotope": str(rng.choice(["18F", "11C", "64Cu", "68Ga"])),
+            "counts": float(10 ** rng.uniform(6, 8)), "beta": float(rng.uniform(0.0, 0.2)), "rho": rho, "truth": act}
+
+
+def uneven_batch(n=12, seed=1004):
+    """The uneven batch: n random volumes from one seed."""
+    rng = np.random.default_rng(seed)
+    return [random_volume(rng) for _ in range(n)]
+
+
+def clinical_thorax(shape=(127, 344, 344), spacing=(2.04, 2.04, 2.03), seed=1005):
+    """Clinical-scale thorax. Returns (rho, truth)."""
+    z, y, x = _grid(shape, spacing)
+    rng = np.random.default_rng(seed)
+    ax, ay = 165.0, 115.0  # body ellipse semi-axes [mm]; y grows towards the back
+    body = (x / ax) ** 2 + (y / ay) ** 2 < 1.0
+    rho = np.where(body, np.float32(1.0), AIR).astype(np.float32)
+    act = body.astype(np.float32)
+    for side in (-1.0, 1.0):
+        lung = _ellipsoid(z, y, x, (10.0, -5.0, side * 72.0), (105.0, 78.0, 55.0)) & body
+        rho[lung] = np.float32(rng.uniform(0.26, 0.32))
+        act[lung] = 0.3
+    # Ribs: bands of the elliptical shell 8-16 mm inside the skin, 10 mm wide every 25 mm in z,
+    # over the sides and the back; the spine: a cylinder behind the mediastinum.
+    shell = ((x / (ax - 8.0)) ** 2 + (y / (ay - 8.0)) ** 2 < 1.0) & ((x / (ax - 16.0)) ** 2 + (y / (ay - 16.0)) ** 2 >= 1.0)
+    ribs = shell & (np.mod(z + 200.0, 25.0) < 10.0) & (y > -0.5 * ay)
+    spine = x ** 2 + (y - 80.0) ** 2 < 16.0 ** 2
+    rho[(ribs | spine) & body] = np.float32(1.85)
+    # Heart: ellipsoid left of the midline, in front; its 9 mm wall at 20x background.
+    centre, semi = (-20.0, -35.0, 28.0), (50.0, 45.0, 55.0)
+    outer = _ellipsoid(z, y, x, centre, semi)
+    inner = _ellipsoid(z, y, x, centre, tuple(s - 9.0 for s in semi))
+    act[outer & ~inner & body] = 20.0
+    act[inner & body] = 2.0  # blood pool
+    placed = 0
+    while placed < 10:  # lesions of radius 3-12 mm at 3-8:1, inside the body
+        r = rng.uniform(3.0, 12.0)
+        cx, cy, cz = rng.uniform(-ax, ax), rng.uniform(-ay, ay), rng.uniform(-110.0, 110.0)
+        if (cx / (ax - r - 10)) ** 2 + (cy / (ay - r - 10)) ** 2 >= 1.0:
+            continue
+        act[((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2 < r * r) & body] = np.float32(rng.uniform(3.0, 8.0))
+        placed += 1
+    return rho, act
