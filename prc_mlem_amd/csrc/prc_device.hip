@@ -220,10 +220,12 @@ struct Scal {
     double hh, hy;       // Biggs-Andrews dot products
     float eps, lambda, scale, beta, gamma;
     int accel;
-    // Power of two applied to the FP16 copy of the forward's input (mixed form): it puts the
-    // data maximum in [256, 512), so that the copy neither overflows nor loses small values;
-    // the forward's output is scaled back exactly in k_ratio.
-    float hscale;
+    // Mixed form: powers of two applied to the FP16 copies that the projector passes gather, set
+    // every iteration from the vector's largest value so that it lands in [2^13, 2^14): the copy
+    // can neither overflow (65504) nor drop values above 2^-28 of its maximum into subnormals.
+    // hscale scales f / W (the forward's output is scaled back exactly in k_ratio), rscale the
+    // ratio (the adjoint's output is scaled back exactly by rinv = 1 / rscale in k_update).
+    float hscale, rscale, rinv;
 };
 
 // Loads and stores through the global address space. Pointers kept in descriptors are generic
@@ -1770,21 +1772,35 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
     const size_t i = V.vbase + loc;                                            \
     (void)i;
 
-// g = f / W for the forward (ForwardOp::begin).
-// gh, when given, receives the FP16 copy that the mixed form's forward gathers.
-// Nonnegative value to FP16, round to nearest, saturated at the largest finite FP16 value: a value
-// past it (a ratio whose forward is almost zero) stays finite in the gathers.
-__device__ inline __half to_half_sat(float v) { return __float2half_rn(fminf(v, 65504.0f)); }
+// Largest value of v over the wave's lanes.
+__device__ inline float wave_max_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
 
-// gh, when given, receives the FP16 copy (times the volume's hscale) that the mixed form's
-// forward gathers.
+// Per-item partial maximum of the values a kernel writes (mixed form), for k_reduce_volume.
+#define ITEM_MAX(v)                                                                              \
+    if (pmax) {                                                                                  \
+        const float m_ = wave_max_f(lane < it.w ? (v) : 0.0f);                                   \
+        if (lane == 0) pmax[item_id] = (double)m_;                                               \
+    }
+
+// g = f / W for the forward (ForwardOp::begin). pmax, when given, receives per-item maxima of g
+// (the mixed form scales its FP16 copy by them).
 __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
-                         const Scal* __restrict__ sc, const float* __restrict__ f, float* __restrict__ g,
-                         __half* __restrict__ gh) {
-    ITEM_PROLOGUE
-    const float v = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
-    g[i] = v;
-    if (gh) gh[i] = to_half_sat(v * sc[it.x].hscale);
+                         const float* __restrict__ f, float* __restrict__ g, double* __restrict__ pmax) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    float v = 0.0f;
+    if (lane < it.w) {
+        const size_t loc = (size_t)it.y * V.nx + it.z + lane, i = V.vbase + loc;
+        v = gload(V.support + loc) ? forward_scale(f[i], gload(V.W + loc)) : 0.0f;
+        g[i] = v;
+    }
+    ITEM_MAX(v)
 }
 
 // Biggs-Andrews prediction y = max(f + lambda (f - prev), f / 2) (prc.cu:318-321), fused with
@@ -1792,29 +1808,56 @@ __global__ void k_fscale(const Vol* __restrict__ vols, const int4* __restrict__ 
 __global__ void k_extrapolate(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
                               const Scal* __restrict__ sc, const float* __restrict__ f,
                               const float* __restrict__ prev, float* __restrict__ y, float* __restrict__ g,
-                              __half* __restrict__ gh) {
-    ITEM_PROLOGUE
-    const float lambda = sc[it.x].lambda;
-    const float fi = f[i];
-    const float yi = fmaxf(fmaf(lambda, fi - prev[i], fi), 0.5f * fi);
-    y[i] = yi;
-    const float v = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
-    g[i] = v;
-    if (gh) gh[i] = to_half_sat(v * sc[it.x].hscale);
+                              double* __restrict__ pmax) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    float v = 0.0f;
+    if (lane < it.w) {
+        const size_t loc = (size_t)it.y * V.nx + it.z + lane, i = V.vbase + loc;
+        const float lambda = sc[it.x].lambda;
+        const float fi = f[i];
+        const float yi = fmaxf(fmaf(lambda, fi - prev[i], fi), 0.5f * fi);
+        y[i] = yi;
+        v = gload(V.support + loc) ? forward_scale(yi, gload(V.W + loc)) : 0.0f;
+        g[i] = v;
+    }
+    ITEM_MAX(v)
 }
 
 // ratio = support ? data / max(est, eps) : 0 (prc.cu:270-274), in place. When the forward read
 // the FP16 copy (`scaled`), est is hscale times the forward and is scaled back first (exactly:
-// hscale is a power of two).
-// rh, when given, receives the FP16 copy that the mixed form's adjoint gathers.
+// hscale is a power of two). pmax, when given, receives per-item maxima of the ratio.
 __global__ void k_ratio(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
                         const Scal* __restrict__ sc, const float* __restrict__ data, float* __restrict__ est,
-                        __half* __restrict__ rh, int scaled) {
+                        double* __restrict__ pmax, int scaled) {
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    if (item_id >= n_items) return;
+    const int4 it = items[item_id];
+    const int lane = threadIdx.x & (kWave - 1);
+    const Vol& V = vols[it.x];
+    float r = 0.0f;
+    if (lane < it.w) {
+        const size_t loc = (size_t)it.y * V.nx + it.z + lane, i = V.vbase + loc;
+        const float e = scaled ? est[i] / sc[it.x].hscale : est[i];
+        r = gload(V.support + loc) ? data[i] / fmaxf(e, sc[it.x].eps) : 0.0f;
+        est[i] = r;
+    }
+    ITEM_MAX(r)
+}
+
+// FP16 copy of a nonnegative vector times the volume's scale (which = 0: hscale, 1: rscale),
+// rounded to nearest; saturation at the largest finite FP16 value only guards the bound.
+__device__ inline __half to_half_sat(float v) { return __float2half_rn(fminf(v, 65504.0f)); }
+
+__global__ void k_half_copy(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
+                            const Scal* __restrict__ sc, const float* __restrict__ src, __half* __restrict__ dst,
+                            int which) {
     ITEM_PROLOGUE
-    const float e = scaled ? est[i] / sc[it.x].hscale : est[i];
-    const float r = gload(V.support + loc) ? data[i] / fmaxf(e, sc[it.x].eps) : 0.0f;
-    est[i] = r;
-    if (rh) rh[i] = to_half_sat(r);
+    const float scale = which ? sc[it.x].rscale : sc[it.x].hscale;
+    dst[i] = to_half_sat(src[i] * scale);
 }
 
 // RDP MAP-EM surrogate update (prc.cu:279-309). Writes next and per-item FP64 partial sums.
@@ -1833,7 +1876,7 @@ __global__ void k_update(const Vol* __restrict__ vols, const int4* __restrict__ 
         const size_t loc = (size_t)row * V.nx + xi;
         const size_t idx = V.vbase + loc;
         if (gload(V.support + loc)) {
-            const float fj = f[idx], f_em = fj * back[idx];
+            const float fj = f[idx], f_em = fj * (back[idx] * S.rinv);  // rinv: exact power of two
             if (S.beta == 0.0f) {
                 result = fmaxf(f_em, 0.0f);
             } else {
@@ -1922,22 +1965,41 @@ __global__ void k_finish(const Vol* __restrict__ vols, const int4* __restrict__ 
 // mode 1: prior count normalisation    scale = sum > 0 ? support_sum / sum : 1 (beta > 0 only)
 // mode 2: Biggs-Andrews lambda from (hy, hh) pairs
 // mode 3: plain sum into Scal::sum
+// mode 4, 5: largest item maximum -> hscale (4) or rscale and rinv (5), powers of two that put it
+//            in [2^13, 2^14) (1 when it is 0)
 __global__ void k_reduce_volume(const int* __restrict__ item_begin, const double* __restrict__ partial,
                                 Scal* __restrict__ sc, int mode) {
     __shared__ double sh[2][256];
     const int v = blockIdx.x;
     const int b = item_begin[v], e = item_begin[v + 1];
     const int stride = mode == 2 ? 2 : 1;
+    const bool is_max = mode >= 4;
     double a0 = 0.0, a1 = 0.0;
     for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
-        a0 += partial[(size_t)stride * i];
+        const double p = partial[(size_t)stride * i];
+        a0 = is_max ? fmax(a0, p) : a0 + p;
         if (mode == 2) a1 += partial[(size_t)stride * i + 1];
     }
     sh[0][threadIdx.x] = a0, sh[1][threadIdx.x] = a1;
     __syncthreads();
     for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[0][threadIdx.x] += sh[0][threadIdx.x + o], sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] = is_max ? fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + o])
+                                        : sh[0][threadIdx.x] + sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
         __syncthreads();
+    }
+    if (is_max) {
+        if (threadIdx.x) return;
+        // m = mant * 2^ex with mant in [0.5, 1): m * 2^(14 - ex) lies in [2^13, 2^14).
+        int ex = 0;
+        const double m = sh[0][0];
+        (void)frexp(m, &ex);
+        const float scale = m > 0.0 ? ldexpf(1.0f, min(100, max(-100, 14 - ex))) : 1.0f;
+        if (mode == 4) sc[v].hscale = scale;
+        else sc[v].rscale = scale, sc[v].rinv = 1.0f / scale;
+        return;
     }
     if (threadIdx.x) return;
     Scal& S = sc[v];
@@ -2788,20 +2850,31 @@ void batch_initialise(const Batch& B) {
 void batch_iteration(const Batch& B) {
     const unsigned nb = blocks_for(B.n_all, kItemsPerBlock);
     const float* start = B.f;
+    // Mixed form: per-item maxima of the gathered vectors go to B.partial, then each volume's
+    // scale, then the FP16 copies.
+    double* pmax = B.gh ? B.partial : nullptr;
     {
         ProfScope ps("extrapolate", B.stream);
         if (B.accel) {
-            k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf, B.gh);
+            k_extrapolate<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.prev, B.y, B.gf, pmax);
             start = B.y;
         } else {
-            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.f, B.gf, B.gh);
+            k_fscale<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.f, B.gf, pmax);
+        }
+        if (B.gh) {
+            k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 4);
+            k_half_copy<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.gf, B.gh, 0);
         }
     }
     run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
                 B.fwd_sched, B.stream);
     {
         ProfScope ps("ratio", B.stream);
-        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, B.rh, B.gh != nullptr);
+        k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, pmax, B.gh != nullptr);
+        if (B.rh) {
+            k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 5);
+            k_half_copy<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.est, B.rh, 1);
+        }
     }
     run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev,
                 B.queue, B.adj_sched, B.stream);
@@ -2909,9 +2982,8 @@ double preprocess_data(const prc_projector* A, const float* host_g, float* out, 
 
 Scal initial_scalars(const prc_projector* A, double support_sum, float vmax, const prc_solve_params& p) {
     Scal sc{};
-    int e = 0;
-    (void)std::frexp(vmax, &e);  // vmax = m 2^e, m in [0.5, 1): vmax 2^(9 - e) in [256, 512)
-    sc.hscale = std::ldexp(1.0f, std::min(100, std::max(-100, 9 - e)));
+    (void)vmax;
+    sc.hscale = sc.rscale = sc.rinv = 1.0f;  // the mixed form sets its scales every iteration
     sc.support_sum = support_sum;
     sc.eps = (float)(1.0e-6 * support_sum / (double)A->stats.n_active);  // prc.cu:668
     sc.lambda = 0.0f, sc.scale = 1.0f, sc.beta = p.beta, sc.gamma = p.gamma;
@@ -3117,8 +3189,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
         float* g = A->scratch_in + A->margin;  // f / W with the forward's gather margin
         {
             ProfScope ps("extrapolate", 0);
-            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, nullptr, dev_in, g,
-                                                                    nullptr);
+            k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
             HIPCHK(hipGetLastError());
         }
         run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);

@@ -86,10 +86,17 @@ def test_water_kernel_matches_exact_voxel_average(ws, oracle_mod, size):
         assert abs((P * axis).sum()) < 1e-6
 
 
-def test_sparse_observations_do_not_lock_supported_zeros(ws):
+PRECISIONS = [(), ("--precision", "mixed")]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_sparse_observations_do_not_lock_supported_zeros(ws, precision):
     sparse = ws["x"].copy()
     sparse[np.random.default_rng(21).random(SHAPE) < 0.5] = 0
-    out = run_prc(ws, sparse, "correct", "--iterations", "5", "--beta", "0.1")
+    out = run_prc(ws, sparse, "correct", "--iterations", "5", "--beta", "0.1", *precision)
+    if precision:  # the mixed form against FP32 on the same data
+        ref = run_prc(ws, sparse, "correct", "--iterations", "5", "--beta", "0.1")
+        assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-3
     sup = ws["support"]
     assert np.mean(out[(sparse == 0) & sup] > 0) > 0.95
     assert out.min() >= 0
@@ -147,7 +154,8 @@ def test_unregularized_rl_conserves_counts(ws, accel):
     assert abs(out.sum(dtype=np.float64) / ws["x"].sum(dtype=np.float64) - 1) <= 1e-6
 
 
-def test_long_sparse_cold_reconstruction_stays_finite_and_preserves_counts(ws):
+@pytest.mark.parametrize("precision", PRECISIONS)
+def test_long_sparse_cold_reconstruction_stays_finite_and_preserves_counts(ws, precision):
     sup = ws["support"]
     truth = sup.astype(np.float32)
     cold = np.zeros(SHAPE, bool)
@@ -159,7 +167,11 @@ def test_long_sparse_cold_reconstruction_stays_finite_and_preserves_counts(ws):
     missed = (sparse == 0) & sup & ~cold
     assert missed.sum() > 20
     for accel in ((), ("--no-accel",)):
-        out = run_prc(ws, sparse, "correct", "--iterations", "30", "--beta", "0.2", "--gamma", "20", *accel)
+        out = run_prc(ws, sparse, "correct", "--iterations", "30", "--beta", "0.2", "--gamma", "20", *accel,
+                      *precision)
+        if precision:
+            ref = run_prc(ws, sparse, "correct", "--iterations", "30", "--beta", "0.2", "--gamma", "20", *accel)
+            assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-3
         assert out.min() >= 0
         np.testing.assert_array_equal(out[~sup], 0)
         assert np.mean(out[missed] > 0) > 0.95
