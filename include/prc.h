@@ -100,6 +100,11 @@ int prc_isotope_find(const char* name); /* index or -1 */
  * on the renormalised model (prc.cu:83-91). */
 float prc_range_cutoff(int isotope, float fraction);
 
+/* The far ray set of prc.cu:118-129 as the device computes it (the march tables are built from
+ * it): start offsets in voxels (3 floats per ray) and unit directions (3 floats per ray) of rays
+ * 0 .. n - 1 of an n-ray set; n even, 2 <= n <= 2^26. For validation. */
+int prc_ray_set(int n, float* host_start, float* host_dir);
+
 /* Build the projector of one volume: density upload (clamped to >= 1e-3, prc.cu:603),
  * device range table, ray tables, and the normalisation pass that stores every
  * system-matrix weight once (replaces Projector::Projector, prc.cu:405-422).

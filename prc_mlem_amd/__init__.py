@@ -64,6 +64,7 @@ SIGNATURES = {
     "prc_isotope_name": (C.c_char_p, [C.c_int]),
     "prc_isotope_find": (C.c_int, [C.c_char_p]),
     "prc_range_cutoff": (C.c_float, [C.c_int, C.c_float]),
+    "prc_ray_set": (C.c_int, [C.c_int, _F32P, _F32P]),
     "prc_projector_create": (C.c_int, [C.POINTER(Geometry), _F32P, C.c_void_p, C.POINTER(C.c_void_p)]),
     "prc_projector_destroy": (None, [C.c_void_p]),
     "prc_projector_normalise": (C.c_int, [C.c_void_p]),
@@ -142,6 +143,14 @@ def isotope_index(name):
 def range_cutoff(isotope, fraction=0.9995):
     """Water-equivalent range cutoff in mm (prc.cu:83-91)."""
     return lib().prc_range_cutoff(isotope_index(isotope), fraction)
+
+
+def ray_set(n):
+    """Start offsets (voxels) and unit directions of the n-ray set as the device computes them,
+    each (n, 3) float32 (prc.cu:118-129)."""
+    start, direction = np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32)
+    _check(lib().prc_ray_set(int(n), start, direction))
+    return start, direction
 
 
 def _volume(a, shape=None):

@@ -368,6 +368,16 @@ __device__ void ray_dir(int k, int n, float3* start, float3* dir) {
                          sign * fmaf(to_unit, (float)(0x80000000u + m * 2360945575u), -0.5f));
 }
 
+// The ray set itself, for validation: start offsets then directions of rays 0 .. n - 1.
+__global__ void k_ray_set(int n, float* __restrict__ out) {
+    const int id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= n) return;
+    float3 x, d;
+    ray_dir(id, n, &x, &d);
+    float* o = out + (size_t)6 * id;
+    o[0] = x.x, o[1] = x.y, o[2] = x.z, o[3] = d.x, o[4] = d.y, o[5] = d.z;
+}
+
 // Geometric voxel walk of one ray (prc.cu:137-163 without the density): the voxel offset and
 // the in-voxel path length of every step are the same for all sources.
 struct Walk {
@@ -3123,6 +3133,25 @@ int prc_isotope_find(const char* name) {
 float prc_range_cutoff(int isotope, float fraction) {
     if (isotope < 0 || isotope >= 8) return NAN;
     return range_cutoff_host(renormalised(isotope), fraction);
+}
+
+int prc_ray_set(int n, float* host_start, float* host_dir) {
+    return guarded([&] {
+        if (n < 2 || (n & 1) || n > (1 << 26) || !host_start || !host_dir) fail(PRC_ERR_INVALID, "invalid argument");
+        float* d = dalloc<float>((size_t)6 * n);
+        std::vector<float> h((size_t)6 * n);
+        try {
+            k_ray_set<<<blocks_for(n, 256), 256>>>(n, d);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        } catch (...) {
+            dfree(d);
+            throw;
+        }
+        dfree(d);
+        for (size_t k = 0; k < (size_t)n; ++k)
+            for (int c = 0; c < 3; ++c) host_start[3 * k + c] = h[6 * k + c], host_dir[3 * k + c] = h[6 * k + 3 + c];
+    });
 }
 
 int prc_projector_create(const prc_geometry* geometry, const float* host_rho, const float* host_table,

@@ -436,3 +436,20 @@ def test_mixed_form_keeps_a_wide_dynamic_range(prc):
         far[v[0] - 5:v[0] + 6, v[1] - 5:v[1] + 6, v[2] - 5:v[2] + 6] = False
     assert far.sum() > 10000
     assert rel_l2(out[far], ref[far]) <= 1e-3
+
+
+def test_device_ray_set_matches_closed_forms_and_oracle(prc, oracle_mod):
+    """The device's ray set (prc.cu:118-129): against the FP64 closed forms of the multipliers at
+    pair indices up to 2^20, where a one-digit change of any multiplier moves offsets or azimuths by
+    more than 1e-4, and against the oracle's FP32 rays."""
+    from test_oracle import closed_form_rays
+    for n in (2, 64, 2048, 1 << 21):
+        announce(f"device ray set n={n}")
+        start, direction = prc.ray_set(n)
+        ks = np.unique(np.concatenate([np.arange(min(n, 8)), np.random.default_rng(n).integers(0, n, 2000)]))
+        want_s, want_d = closed_form_rays(ks, n)
+        np.testing.assert_allclose(start[ks], want_s, rtol=0, atol=1e-7)
+        np.testing.assert_allclose(direction[ks], want_d, rtol=0, atol=4e-7)
+        o = [oracle_mod.ray_f32(int(k), n) for k in ks[:200]]
+        np.testing.assert_allclose(start[ks[:200]], np.array([v[0] for v in o]), rtol=0, atol=6e-8)
+        np.testing.assert_allclose(direction[ks[:200]], np.array([v[1] for v in o]), rtol=0, atol=6e-8)
