@@ -126,6 +126,13 @@ int prc_projectors_normalise(int n, prc_projector* const* projectors);
 
 /* dev_out = A dev_in, overwriting every voxel (Projector::forward, prc.cu:430-434). */
 int prc_projector_forward(prc_projector* projector, const float* dev_in, float* dev_out);
+/* The same as prc_projector_forward, allocating nothing. No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). The staging vector f / W lives in a static device array of
+ * 32 Mi floats (128 MiB, one per GPU, held until the process ends) instead of a buffer allocated on
+ * first use; a volume that needs more (voxels plus two gather margins) is rejected with
+ * PRC_ERR_INVALID. Calls that overlap on one GPU share that array and race. */
+int prc_projector_forward_noalloc(prc_projector* projector, const float* dev_in, float* dev_out);
 /* dev_out = A^T dev_in on active voxels; other voxels are left untouched
  * (Projector::adjoint, prc.cu:436). */
 int prc_projector_adjoint(prc_projector* projector, const float* dev_in, float* dev_out);

@@ -70,6 +70,7 @@ SIGNATURES = {
     "prc_projector_normalise": (C.c_int, [C.c_void_p]),
     "prc_projectors_normalise": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
     "prc_projector_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "prc_projector_forward_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "prc_projector_forward_host": (C.c_int, [C.c_void_p, _F32P, _F32P]),
     "prc_projector_adjoint_host": (C.c_int, [C.c_void_p, _F32P, _F32P]),
@@ -206,6 +207,10 @@ class Projector:
     def forward_device(self, x_ptr, out_ptr):
         """A x on device pointers (e.g. torch tensors' data_ptr()); stream-ordered on the null stream."""
         _check(lib().prc_projector_forward(self.handle, C.c_void_p(x_ptr), C.c_void_p(out_ptr)))
+
+    def forward_device_noalloc(self, x_ptr, out_ptr):
+        """A x on device pointers, allocating nothing (the staging vector is a static device array)."""
+        _check(lib().prc_projector_forward_noalloc(self.handle, C.c_void_p(x_ptr), C.c_void_p(out_ptr)))
 
     def adjoint_device(self, y_ptr, out_ptr):
         _check(lib().prc_projector_adjoint(self.handle, C.c_void_p(y_ptr), C.c_void_p(out_ptr)))

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstring>
 #include <type_traits>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -3117,6 +3118,40 @@ void batch_run(prc_batch* b, int iterations) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------
+// Forward without allocation (prc_projector_forward_noalloc): the margin-padded staging vector
+// g = f / W lives in this file-scope array instead of a buffer the library allocates on first
+// use. Each GPU holds its own copy until the process ends. Host code never uses the array's name
+// as an address: scratch_base() looks the device address up once per device.
+constexpr size_t kScratchFloats = size_t(32) << 20;  // 128 MiB: n + 2 margins of a 344x344x127 volume
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+float* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+float* scratch_base() {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(PRC_ERR_DEVICE, "device %d beyond the library's %d", dev, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (float*)p;
+    }
+    return g_scratch[dev];
+}
+
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace
+
 // =======================================================================================
 extern "C" {
 
@@ -3221,6 +3256,28 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
             HIPCHK(hipGetLastError());
         }
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
+    });
+}
+
+int prc_projector_forward_noalloc(prc_projector* A, const float* dev_in, float* dev_out) {
+    return guarded([&] {
+        if (!A || !dev_in || !dev_out) fail(PRC_ERR_INVALID, "null argument");
+        const size_t margin = (size_t)A->margin, need = A->n + 2 * margin;
+        if (need > kScratchFloats)
+            fail(PRC_ERR_INVALID,
+                 "the no-allocation forward holds %zu floats of scratch; this volume needs %zu (voxels plus two "
+                 "gather margins): use prc_projector_forward",
+                 kScratchFloats, need);
+        float* base = scratch_base();
+        // The margins are read by the gathers (against zero weights): clear them, with a kernel so
+        // that the call can be captured in a graph. The volume part is written whole by k_fscale.
+        const unsigned fb = std::min<unsigned>(blocks_for((long long)margin, 256), 1024u);
+        k_fill<<<fb, 256>>>(base, (long long)margin, 0.0f);
+        k_fill<<<fb, 256>>>(base + margin + A->n, (long long)margin, 0.0f);
+        float* g = base + margin;
+        k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
+        HIPCHK(hipGetLastError());
         run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
     });
 }
