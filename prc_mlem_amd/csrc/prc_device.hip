@@ -1666,6 +1666,25 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
 enum Sched { kSchedGrid = 0, kSchedQueue = 1, kSchedChunked = 2 };
 constexpr int kSchedCount = 3;
 
+// A pass's launch shape, chosen per solver or batch by tune_passes: the schedule (enum Sched) and,
+// for the queue schedule, its persistent grid in blocks per CU.
+struct PassShape {
+    int sched = kSchedQueue;
+    int blocks_per_cu = 2;
+};
+const PassShape kShapes[] = {
+    {kSchedGrid, 0}, {kSchedChunked, 0}, {kSchedQueue, 1}, {kSchedQueue, 2}, {kSchedQueue, 4},
+};
+constexpr int kShapeCount = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
+constexpr int kDefaultShape = 3;  // queue, two blocks per CU
+std::string shape_name(const PassShape& p) {
+    static const char* const sched[kSchedCount] = {"grid", "queue", "chunked"};
+    std::string n = sched[p.sched];
+    if (p.sched == kSchedQueue) n += "@" + std::to_string(p.blocks_per_cu);
+    return n;
+}
+
+
 __device__ inline unsigned chunked_block() {
     const unsigned nb = gridDim.x, b = blockIdx.x, x = b % kXcds, j = b / kXcds;
     const unsigned q = nb / kXcds, r = nb % kXcds;  // XCD x runs q + (x < r) blocks
@@ -2187,7 +2206,7 @@ struct prc_projector {
     // Pinned staging of the sizes normalise_begin reads back (so the read-back does not block
     // the host): total[4] (long long), then the memo counts [2] and kglob [n_rays + 1] (int).
     void* h_stage = nullptr;
-    int fwd_sched = 1, adj_sched = 1;  // projector-pass schedules (tuned with the first solver)
+    PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // tuned with the first solver
     bool counted = false;  // march statistics read (after the first normalisation)
     bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
@@ -2759,57 +2778,59 @@ void release(prc_projector* A) {
 // The gathered input is FP32, or FP16 (`half_in`, the mixed-form solver's iterates) with FP16
 // weights.
 template <class WT, class VT>
-void launch_forward(const Vol* dv, const int4* items, int n_items, const void* g, float* out, int* queue, int sched,
-                    unsigned nb, hipStream_t s) {
+void launch_forward(const Vol* dv, const int4* items, int n_items, const void* g, float* out, int* queue,
+                    const PassShape& sh, unsigned nb, hipStream_t s) {
     const VT* in = (const VT*)g;
-    if (sched == kSchedQueue) k_forward<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    else if (sched == kSchedChunked) k_forward<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    if (sh.sched == kSchedQueue) k_forward<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else if (sh.sched == kSchedChunked) k_forward<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
     else k_forward<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
 }
 template <class WT, class VT>
-void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* in, float* out, int* queue, int sched,
-                    unsigned nb, hipStream_t s) {
+void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* in, float* out, int* queue,
+                    const PassShape& sh, unsigned nb, hipStream_t s) {
     const VT* v = (const VT*)in;
-    if (sched == kSchedQueue) k_adjoint<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
-    else if (sched == kSchedChunked) k_adjoint<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    if (sh.sched == kSchedQueue) k_adjoint<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    else if (sh.sched == kSchedChunked) k_adjoint<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
     else k_adjoint<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
 }
 
-// Persistent grid of the queue schedule: two blocks (eight waves) per CU, a quarter of the waves
-// that fit. Fewer resident waves keep each XCD's concurrent gathers within fewer rows (measured
-// faster on the mixed passes than a full grid) and leave wave slots to the kernels of another
-// stream running beside the pass.
-unsigned queue_blocks() {
-    static const unsigned nb = [] {
-        int dev = 0, cus = 0;
+// Persistent grid of the queue schedule: blocks_per_cu blocks (four waves each) per CU. Fewer
+// resident waves keep each XCD's concurrent gathers within fewer rows and leave wave slots to the
+// kernels of another stream running beside the pass; the tuner picks the count.
+unsigned device_cus() {
+    static const unsigned cus = [] {
+        int dev = 0, n = 0;
         HIPCHK(hipGetDevice(&dev));
-        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        return 2u * (unsigned)std::max(cus, 1);
+        HIPCHK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+        return (unsigned)std::max(n, 1);
     }();
-    return nb;
+    return cus;
+}
+
+unsigned pass_blocks(const PassShape& sh, int n_items) {
+    const unsigned all = blocks_for(n_items, kItemsPerBlock);
+    return sh.sched == kSchedQueue ? std::min<unsigned>((unsigned)sh.blocks_per_cu * device_cus(), all) : all;
 }
 
 void run_forward(const Vol* dv, const int4* items, int n_items, int wbytes, const void* g, bool half_in, float* out,
-                 int* queue, int sched, hipStream_t s) {
-    const unsigned all = blocks_for(n_items, kItemsPerBlock);
-    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(queue_blocks(), all) : all;
-    if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
+                 int* queue, const PassShape& sh, hipStream_t s) {
+    const unsigned nb = pass_blocks(sh, n_items);
+    if (sh.sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("forward", s);
-    if (wbytes == 4) launch_forward<float, float>(dv, items, n_items, g, out, queue, sched, nb, s);
-    else if (half_in) launch_forward<__half, __half>(dv, items, n_items, g, out, queue, sched, nb, s);
-    else launch_forward<__half, float>(dv, items, n_items, g, out, queue, sched, nb, s);
+    if (wbytes == 4) launch_forward<float, float>(dv, items, n_items, g, out, queue, sh, nb, s);
+    else if (half_in) launch_forward<__half, __half>(dv, items, n_items, g, out, queue, sh, nb, s);
+    else launch_forward<__half, float>(dv, items, n_items, g, out, queue, sh, nb, s);
     HIPCHK(hipGetLastError());
 }
 
 void run_adjoint(const Vol* dv, const int4* items, int n_items, int wbytes, const void* in, bool half_in, float* out,
-                 int* queue, int sched, hipStream_t s) {
-    const unsigned all = blocks_for(n_items, kItemsPerBlock);
-    const unsigned nb = sched == kSchedQueue ? std::min<unsigned>(queue_blocks(), all) : all;
-    if (sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
+                 int* queue, const PassShape& sh, hipStream_t s) {
+    const unsigned nb = pass_blocks(sh, n_items);
+    if (sh.sched == kSchedQueue) k_reset_queue<<<1, 64, 0, s>>>(queue);
     ProfScope ps("adjoint", s);
-    if (wbytes == 4) launch_adjoint<float, float>(dv, items, n_items, in, out, queue, sched, nb, s);
-    else if (half_in) launch_adjoint<__half, __half>(dv, items, n_items, in, out, queue, sched, nb, s);
-    else launch_adjoint<__half, float>(dv, items, n_items, in, out, queue, sched, nb, s);
+    if (wbytes == 4) launch_adjoint<float, float>(dv, items, n_items, in, out, queue, sh, nb, s);
+    else if (half_in) launch_adjoint<__half, __half>(dv, items, n_items, in, out, queue, sh, nb, s);
+    else launch_adjoint<__half, float>(dv, items, n_items, in, out, queue, sh, nb, s);
     HIPCHK(hipGetLastError());
 }
 
@@ -2836,7 +2857,7 @@ struct Batch {
     const int4* items_fwd = nullptr;
     int n_all = 0, n_act = 0, n_fwd = 0, n_vols = 1, wbytes = 4;
     int* queue = nullptr;  // work-queue counters of the projector passes
-    int fwd_sched = kSchedQueue, adj_sched = kSchedQueue;  // chosen by tune_passes
+    PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // chosen by tune_passes
     const int* item_begin = nullptr;  // per-volume ranges of items_all
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
@@ -2878,7 +2899,7 @@ void batch_iteration(const Batch& B) {
         }
     }
     run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
-                B.fwd_sched, B.stream);
+                B.fwd_shape, B.stream);
     {
         ProfScope ps("ratio", B.stream);
         k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, pmax, B.gh != nullptr);
@@ -2888,7 +2909,7 @@ void batch_iteration(const Batch& B) {
         }
     }
     run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev,
-                B.queue, B.adj_sched, B.stream);
+                B.queue, B.adj_shape, B.stream);
     {
         ProfScope ps("update", B.stream);
         k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
@@ -2903,44 +2924,51 @@ void batch_iteration(const Batch& B) {
     HIPCHK(hipGetLastError());
 }
 
-// Autotuner of the projector passes: every schedule of the forward and the adjoint is timed on
-// the solver's or batch's own weights and buffers (the forward into the estimate buffer, the
-// adjoint into the work buffer; the caller resets both afterwards), and the fastest is kept. The
-// winner depends on the batch: several volumes per launch favour the XCD-local queues, a single
-// volume the launch-order grid. PRC_TUNE=0 keeps the defaults.
+// Autotuner of the projector passes: every launch shape of the forward and the adjoint (schedule
+// and persistent-grid size) is timed on the solver's or batch's own
+// weights and buffers (the forward into the estimate buffer, the adjoint into the work buffer; the
+// caller resets both afterwards), and the fastest is kept. Every shape gives the same result bit
+// for bit (same per-lane summation order). PRC_TUNE=0 keeps the default shape; PRC_TUNE=<name>
+// forces one (names from shape_name; a bare "queue" is the default two blocks per CU).
 struct TuneRecord {
     int vols, wbytes, items_fwd, items_adj;
-    float ms[2][kSchedCount];  // [forward, adjoint][grid, queue, chunked], mean of two launches
+    float ms[2][kShapeCount];  // [forward, adjoint][shape], mean of two launches
     int choice[2];
 };
 std::vector<TuneRecord> g_tuning;
-const char* const kSchedName[kSchedCount] = {"grid", "queue", "chunked"};
 
-// PRC_TUNE=0 keeps the default schedule; PRC_TUNE=grid|queue|chunked forces one (tests).
+int shape_by_name(const char* name) {
+    for (int k = 0; k < kShapeCount; ++k)
+        if (shape_name(kShapes[k]) == name) return k;
+    if (!strcmp(name, "queue")) return kDefaultShape;
+    return -1;
+}
+
 void tune_passes(Batch& B) {
     const char* env = getenv("PRC_TUNE");
     if (env && env[0] == '0') return;
-    for (int sched = 0; env && sched < kSchedCount; ++sched)
-        if (strcmp(env, kSchedName[sched]) == 0) {
-            B.fwd_sched = B.adj_sched = sched;
-            return;
-        }
+    if (env && *env) {
+        const int k = shape_by_name(env);
+        if (k < 0) fail(PRC_ERR_INVALID, "PRC_TUNE=%s names no projector-pass shape", env);
+        B.fwd_shape = B.adj_shape = kShapes[k];
+        return;
+    }
     const bool prof = g_prof_on;
     g_prof_on = false;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {{0, 0, 0}, {0, 0, 0}}, {0, 0}};
+    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {}, {kDefaultShape, kDefaultShape}};
     try {
         for (int op = 0; op < 2; ++op)
-            for (int sched = 0; sched < kSchedCount; ++sched) {
+            for (int k = 0; k < kShapeCount; ++k) {
                 auto launch = [&] {
                     if (op == 0)
                         run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf,
-                                    B.gh != nullptr, B.est, B.queue, sched, B.stream);
+                                    B.gh != nullptr, B.est, B.queue, kShapes[k], B.stream);
                     else
                         run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est,
-                                    B.rh != nullptr, B.prev, B.queue, sched, B.stream);
+                                    B.rh != nullptr, B.prev, B.queue, kShapes[k], B.stream);
                 };
                 launch();  // warm-up
                 HIPCHK(hipEventRecord(e0, B.stream));
@@ -2950,7 +2978,7 @@ void tune_passes(Batch& B) {
                 HIPCHK(hipEventSynchronize(e1));
                 float ms = 0.0f;
                 HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                r.ms[op][sched] = ms / 2;
+                r.ms[op][k] = ms / 2;
             }
     } catch (...) {
         (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
@@ -2959,12 +2987,10 @@ void tune_passes(Batch& B) {
     }
     (void)hipEventDestroy(e0), (void)hipEventDestroy(e1);
     g_prof_on = prof;
-    for (int op = 0; op < 2; ++op) {
-        r.choice[op] = kSchedGrid;
-        for (int sched = 1; sched < kSchedCount; ++sched)
-            if (r.ms[op][sched] < r.ms[op][r.choice[op]]) r.choice[op] = sched;
-    }
-    B.fwd_sched = r.choice[0], B.adj_sched = r.choice[1];
+    for (int op = 0; op < 2; ++op)
+        for (int k = 0; k < kShapeCount; ++k)
+            if (r.ms[op][k] < r.ms[op][r.choice[op]]) r.choice[op] = k;
+    B.fwd_shape = kShapes[r.choice[0]], B.adj_shape = kShapes[r.choice[1]];
     g_tuning.push_back(r);
 }
 
@@ -3256,7 +3282,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
             HIPCHK(hipGetLastError());
         }
-        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_shape, 0);
     });
 }
 
@@ -3278,7 +3304,7 @@ int prc_projector_forward_noalloc(prc_projector* A, const float* dev_in, float* 
         float* g = base + margin;
         k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
         HIPCHK(hipGetLastError());
-        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_sched, 0);
+        run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_shape, 0);
     });
 }
 
@@ -3291,7 +3317,7 @@ int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out)
         }
         float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
         HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
-        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, false, dev_out, A->queue, A->adj_sched, 0);
+        run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, false, dev_out, A->queue, A->adj_shape, 0);
     });
 }
 
@@ -3387,7 +3413,7 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
             if (!A->tuned) {  // the projector's first solver tunes its passes
                 Batch B = solver_batch(S);
                 tune_passes(B);
-                A->fwd_sched = B.fwd_sched, A->adj_sched = B.adj_sched, A->tuned = true;
+                A->fwd_shape = B.fwd_shape, A->adj_shape = B.adj_shape, A->tuned = true;
                 rc = prc_solver_reset(S);
                 if (rc) throw Failure{rc};
             }
@@ -3424,7 +3450,7 @@ static Batch solver_batch(prc_solver* S) {
     prc_projector* A = S->A;
     B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
     B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd, B.queue = A->queue;
-    B.fwd_sched = A->fwd_sched, B.adj_sched = A->adj_sched;
+    B.fwd_shape = A->fwd_shape, B.adj_shape = A->adj_shape;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     if (S->hraw[0]) B.gh = S->hraw[0] + S->A->margin, B.rh = S->hraw[1] + S->A->margin;
@@ -3627,15 +3653,20 @@ int prc_tuning_report(char* json, size_t capacity) {
         std::string s = "[";
         for (size_t i = 0; i < g_tuning.size(); ++i) {
             const TuneRecord& r = g_tuning[i];
-            char buf[512];
-            snprintf(buf, sizeof(buf),
-                     "%s{\"volumes\": %d, \"weight_bytes\": %d, \"forward_items\": %d, \"adjoint_items\": %d, "
-                     "\"forward_ms\": {\"grid\": %.4f, \"queue\": %.4f, \"chunked\": %.4f}, "
-                     "\"adjoint_ms\": {\"grid\": %.4f, \"queue\": %.4f, \"chunked\": %.4f}, "
-                     "\"forward\": \"%s\", \"adjoint\": \"%s\"}",
-                     i ? ", " : "", r.vols, r.wbytes, r.items_fwd, r.items_adj, r.ms[0][0], r.ms[0][1], r.ms[0][2],
-                     r.ms[1][0], r.ms[1][1], r.ms[1][2], kSchedName[r.choice[0]], kSchedName[r.choice[1]]);
+            char buf[256];
+            snprintf(buf, sizeof(buf), "%s{\"volumes\": %d, \"weight_bytes\": %d, \"forward_items\": %d, \"adjoint_items\": %d",
+                     i ? ", " : "", r.vols, r.wbytes, r.items_fwd, r.items_adj);
             s += buf;
+            for (int op = 0; op < 2; ++op) {
+                s += op ? ", \"adjoint_ms\": {" : ", \"forward_ms\": {";
+                for (int k = 0; k < kShapeCount; ++k) {
+                    snprintf(buf, sizeof(buf), "%s\"%s\": %.4f", k ? ", " : "", shape_name(kShapes[k]).c_str(), r.ms[op][k]);
+                    s += buf;
+                }
+                s += "}";
+            }
+            s += ", \"forward\": \"" + shape_name(kShapes[r.choice[0]]) + "\", \"adjoint\": \"" +
+                 shape_name(kShapes[r.choice[1]]) + "\"}";
         }
         s += "]";
         if (!json || capacity < s.size() + 1) fail(PRC_ERR_INVALID, "report needs %zu bytes", s.size() + 1);
