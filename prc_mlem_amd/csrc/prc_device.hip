@@ -1890,55 +1890,113 @@ __global__ void k_half_copy(const Vol* __restrict__ vols, const int4* __restrict
     dst[i] = to_half_sat(src[i] * scale);
 }
 
-// RDP MAP-EM surrogate update (prc.cu:279-309). Writes next and per-item FP64 partial sums.
-__global__ void k_update(const Vol* __restrict__ vols, const int4* __restrict__ items, int n_items,
-                         const Scal* __restrict__ sc, const float* __restrict__ f, const float* __restrict__ back,
-                         float* __restrict__ next, double* __restrict__ partial) {
-    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
-    if (item_id >= n_items) return;
-    const int4 it = items[item_id];
-    const int lane = threadIdx.x & (kWave - 1);
-    const Vol& V = vols[it.x];
-    const Scal& S = sc[it.x];
-    float result = 0.0f;
-    if (lane < it.w) {
-        const int row = it.y, xi = it.z + lane, yi = row % V.ny, zi = row / V.ny;
-        const size_t loc = (size_t)row * V.nx + xi;
-        const size_t idx = V.vbase + loc;
-        if (gload(V.support + loc)) {
-            const float fj = f[idx], f_em = fj * (back[idx] * S.rinv);  // rinv: exact power of two
-            if (S.beta == 0.0f) {
-                result = fmaxf(f_em, 0.0f);
-            } else {
-                const float gamma = S.gamma, eps = S.eps;
-                const float smin = fminf(V.sx, fminf(V.sy, V.sz));
-                float grad = 0.0f, curv = 0.0f;
-                for (int dz = -1; dz <= 1; ++dz)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dx = -1; dx <= 1; ++dx) {
-                            if (!(dx | dy | dz)) continue;
-                            const int a = xi + dx, b = yi + dy, c = zi + dz;
-                            if (a < 0 || a >= V.nx || b < 0 || b >= V.ny || c < 0 || c >= V.nz) continue;
-                            const long nb = (long)loc + ((long)dz * V.ny + dy) * V.nx + dx;
-                            if (!gload(V.support + nb)) continue;
-                            const float mx = dx * V.sx, my = dy * V.sy, mz = dz * V.sz;
-                            const float w = smin / sqrtf(mx * mx + my * my + mz * mz);
-                            const float fk = f[V.vbase + nb], d = fj - fk, ad = fabsf(d);
-                            const float D = fj + fk + gamma * ad + eps, invD = 1.0f / D;
-                            grad += w * d * (fj + 3.0f * fk + gamma * ad + 2.0f * eps) * invD * invD;
-                            const float half_quad = (2.0f * (fj + fk + eps) + gamma * ad) * invD * invD;
-                            const float hess = 2.0f * (2.0f * fk + eps) * (2.0f * fk + eps) * invD * invD * invD;
-                            curv += 2.0f * w * fmaxf(half_quad, hess);
-                        }
-                const float a = S.beta * curv, B = 1.0f + S.beta * (grad - curv * fj);
-                const float root = sqrtf(B * B + 4.0f * a * f_em);
-                result = B >= 0.0f ? (f_em > 0.0f ? 2.0f * f_em / (B + root) : 0.0f) : (root - B) / (2.0f * a);
-            }
-        }
-        next[idx] = result;
+// RDP MAP-EM surrogate update (prc.cu:279-309), fused with the support mask and the FP64 partial
+// sums of the count normalisation. One block of 256 threads updates a tile of 64 (x) x 4 (y)
+// voxels over kTileZ slices, one voxel per thread per slice: it marches along z keeping three
+// planes of f and the support, with a one-voxel halo, in an LDS ring (plane z + 1 is loaded
+// while z - 1 .. z + 1 are read), so each value of f is read from memory 1.5 times instead of up
+// to 27. The 26 distance weights min(s) / |(dx sx, dy sy, dz sz)| are computed once per block.
+// Neighbours are visited in the reference's order (dz, dy, dx), and a neighbour outside the grid
+// or the support (both stored as support 0 in the halo) is skipped, as prc.cu:290-293 does.
+// Writes next on every voxel of the tile (0 off support) and one partial sum per tile.
+constexpr int kTileY = 4;
+constexpr int kTileZ = 4;
+constexpr int kHaloX = kWave + 2, kHaloY = kTileY + 2, kHaloPlane = kHaloX * kHaloY;
+static_assert(kWave * kTileY == 256, "one voxel per thread of a 256-thread block");
+
+__global__ __launch_bounds__(256) void k_update(const Vol* __restrict__ vols, const int4* __restrict__ tiles,
+                                                 const Scal* __restrict__ sc, const float* __restrict__ f,
+                                                 const float* __restrict__ back, float* __restrict__ next,
+                                                 double* __restrict__ partial) {
+    __shared__ float fs[3][kHaloPlane];
+    __shared__ unsigned char ss[3][kHaloPlane];
+    __shared__ float wnb[27];
+    __shared__ double red[256 / kWave];
+    const int tid = threadIdx.x;
+    const int4 t = tiles[blockIdx.x];
+    const Vol& V = vols[t.x];
+    const Scal& S = sc[t.x];
+    const int nx = V.nx, ny = V.ny, nz = V.nz, x0 = t.y, y0 = t.z, z0 = t.w;
+    const bool prior = S.beta != 0.0f;
+    if (tid < 27) {
+        const float smin = fminf(V.sx, fminf(V.sy, V.sz));
+        const float mx = (tid % 3 - 1) * V.sx, my = (tid / 3 % 3 - 1) * V.sy, mz = (tid / 9 - 1) * V.sz;
+        wnb[tid] = tid == 13 ? 0.0f : smin / sqrtf(mx * mx + my * my + mz * mz);
     }
-    const double tot = wave_sum_d((double)result);
-    if (lane == 0) partial[item_id] = tot;
+    // Plane z of the tile with its halo into ring slot `slot` (outside the grid: support 0).
+    auto load_plane = [&](int z, int slot) {
+        for (int e = tid; e < kHaloPlane; e += 256) {
+            const int x = x0 - 1 + e % kHaloX, y = y0 - 1 + e / kHaloX;
+            const bool in = x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            const size_t loc = in ? ((size_t)z * ny + y) * nx + x : 0;
+            const unsigned char su = in ? gload(V.support + loc) : (unsigned char)0;
+            fs[slot][e] = su ? f[V.vbase + loc] : 0.0f;
+            ss[slot][e] = su;
+        }
+    };
+    const int lx = tid & (kWave - 1), ly = tid / kWave;
+    const int x = x0 + lx, y = y0 + ly;
+    const bool mine = x < nx && y < ny;
+    const int c = (ly + 1) * kHaloX + lx + 1;  // the thread's voxel in a halo plane
+    if (prior) {
+        load_plane(z0 - 1, 0);
+        load_plane(z0, 1);
+    }
+    double acc = 0.0;
+    for (int k = 0; k < kTileZ && z0 + k < nz; ++k) {  // the bound is the block's own
+        const int z = z0 + k;
+        const int s_lo = k % 3, s_mid = (k + 1) % 3, s_hi = (k + 2) % 3;
+        if (prior) load_plane(z + 1, s_hi);
+        __syncthreads();  // the weights and this step's planes are in place
+        float result = 0.0f;
+        if (mine) {
+            const size_t loc = ((size_t)z * ny + y) * nx + x, idx = V.vbase + loc;
+            const bool su = prior ? ss[s_mid][c] != 0 : gload(V.support + loc) != 0;
+            if (su) {
+                const float fj = prior ? fs[s_mid][c] : f[idx];
+                const float f_em = fj * (back[idx] * S.rinv);  // rinv: exact power of two
+                if (!prior) {
+                    result = fmaxf(f_em, 0.0f);
+                } else {
+                    const float gamma = S.gamma, eps = S.eps;
+                    float grad = 0.0f, curv = 0.0f;
+#pragma unroll
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        const int slot = dz < 0 ? s_lo : dz > 0 ? s_hi : s_mid;
+#pragma unroll
+                        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                            for (int dx = -1; dx <= 1; ++dx) {
+                                if (!(dx | dy | dz)) continue;
+                                const int h = c + dy * kHaloX + dx;
+                                if (!ss[slot][h]) continue;
+                                const float w = wnb[(dz + 1) * 9 + (dy + 1) * 3 + dx + 1];
+                                const float fk = fs[slot][h], d = fj - fk, ad = fabsf(d);
+                                const float D = fj + fk + gamma * ad + eps, invD = 1.0f / D;
+                                grad += w * d * (fj + 3.0f * fk + gamma * ad + 2.0f * eps) * invD * invD;
+                                const float half_quad = (2.0f * (fj + fk + eps) + gamma * ad) * invD * invD;
+                                const float hess = 2.0f * (2.0f * fk + eps) * (2.0f * fk + eps) * invD * invD * invD;
+                                curv += 2.0f * w * fmaxf(half_quad, hess);
+                            }
+                    }
+                    const float a = S.beta * curv, B = 1.0f + S.beta * (grad - curv * fj);
+                    const float root = sqrtf(B * B + 4.0f * a * f_em);
+                    result = B >= 0.0f ? (f_em > 0.0f ? 2.0f * f_em / (B + root) : 0.0f) : (root - B) / (2.0f * a);
+                }
+            }
+            next[idx] = result;
+        }
+        acc += (double)result;
+        __syncthreads();  // every thread is done with slot s_lo before the next step refills it
+    }
+    acc = wave_sum_d(acc);
+    if (lx == 0) red[ly] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < 256 / kWave; ++w) tot += red[w];
+        partial[blockIdx.x] = tot;
+    }
 }
 
 // f = max(f, floor) on support (prc.cu:311-314) and per-item FP64 partial sums.
@@ -2198,6 +2256,8 @@ struct prc_projector {
     int4* items_act = nullptr;  // active row segments (march, adjoint)
     int4* items_all = nullptr;  // every row in 64-voxel pieces (streaming kernels)
     int4* items_fwd = nullptr;  // active segments plus the zero pieces around them (forward)
+    int4* tiles = nullptr;      // 64 x 4 x kTileZ voxel tiles of the RDP update (vol, x0, y0, z0)
+    int n_tiles = 0;
     int* queue = nullptr;       // work-queue counters of the projector passes
     // A normalisation runs on the library's build stream, or on the stream of the batch that asks
     // for it (prc_batch_normalise). It first waits for the null stream (solvers) and for the
@@ -2210,7 +2270,7 @@ struct prc_projector {
     bool counted = false;  // march statistics read (after the first normalisation)
     bool tuned = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
-    std::vector<int4> h_items_act, h_items_all, h_items_fwd;
+    std::vector<int4> h_items_act, h_items_all, h_items_fwd, h_tiles;
     std::vector<int> h_row_xa, h_row_len;  // active x range of every row
     std::vector<unsigned char> h_support;
     unsigned long long* counters = nullptr;  // vrs_near, vrs_far, deposits
@@ -2242,6 +2302,7 @@ struct prc_solver {
     Scal sc0{};
     double* partial = nullptr;
     int* item_begin = nullptr;
+    int* tile_begin = nullptr;
     double support_sum = 0.0;
     bool initialised = false;
 };
@@ -2301,6 +2362,13 @@ void build_items(prc_projector* A) {
     long long n_rows = 0;
     for (int row = 0; row < rows; ++row) n_rows += len[row] > 0;
     A->stats.n_rows = n_rows;
+    A->h_tiles.clear();
+    for (int z = 0; z < A->g.nz; z += kTileZ)
+        for (int y = 0; y < A->g.ny; y += kTileY)
+            for (int x = 0; x < nx; x += kWave) A->h_tiles.push_back(make_int4(0, x, y, z));
+    A->n_tiles = (int)A->h_tiles.size();
+    A->tiles = dalloc<int4>(A->n_tiles);
+    HIPCHK(hipMemcpy(A->tiles, A->h_tiles.data(), A->n_tiles * sizeof(int4), hipMemcpyHostToDevice));
 }
 
 template <class WT>
@@ -2762,7 +2830,7 @@ void release(prc_projector* A) {
     if (A->h_stage) (void)hipHostFree(A->h_stage), A->h_stage = nullptr;
     dfree(A->dv), dfree(A->rho_alloc), dfree(A->table), dfree(A->support), dfree(A->W), dfree(A->row_xa);
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->fsoa), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
-    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
+    dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->tiles), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
     dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->tmaj), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
@@ -2859,6 +2927,9 @@ struct Batch {
     int* queue = nullptr;  // work-queue counters of the projector passes
     PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // chosen by tune_passes
     const int* item_begin = nullptr;  // per-volume ranges of items_all
+    const int4* tiles = nullptr;      // RDP update tiles of every volume
+    int n_tiles = 0;
+    const int* tile_begin = nullptr;  // per-volume ranges of tiles
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
     __half *gh = nullptr, *rh = nullptr;  // mixed form: FP16 copies of the forward and adjoint inputs
@@ -2912,8 +2983,8 @@ void batch_iteration(const Batch& B) {
                 B.queue, B.adj_shape, B.stream);
     {
         ProfScope ps("update", B.stream);
-        k_update<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, start, B.prev, B.est, B.partial);
-        if (B.any_prior) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.item_begin, B.partial, B.sc, 1);
+        k_update<<<B.n_tiles, 256, 0, B.stream>>>(B.dv, B.tiles, B.sc, start, B.prev, B.est, B.partial);
+        if (B.any_prior) k_reduce_volume<<<B.n_vols, 256, 0, B.stream>>>(B.tile_begin, B.partial, B.sc, 1);
     }
     if (B.any_prior || B.accel) {
         ProfScope ps("finish", B.stream);
@@ -3404,10 +3475,12 @@ int prc_solver_create(prc_projector* A, const float* host_g, const prc_solve_par
             for (int i = 0; i < 5; ++i) S->base[i] = S->raw[i] + A->margin;
             HIPCHK(hipMemcpy(S->g, g.data(), n * sizeof(float), hipMemcpyHostToDevice));
             S->sc = dalloc<Scal>(1);
-            S->partial = dalloc<double>(2 * (size_t)A->n_all);
-            int ib[2] = {0, A->n_all};
+            S->partial = dalloc<double>(std::max<size_t>(2 * (size_t)A->n_all, (size_t)A->n_tiles));
+            int ib[2] = {0, A->n_all}, tb[2] = {0, A->n_tiles};
             S->item_begin = dalloc<int>(2);
             HIPCHK(hipMemcpy(S->item_begin, ib, sizeof(ib), hipMemcpyHostToDevice));
+            S->tile_begin = dalloc<int>(2);
+            HIPCHK(hipMemcpy(S->tile_begin, tb, sizeof(tb), hipMemcpyHostToDevice));
             int rc = prc_solver_reset(S);
             if (rc) throw Failure{rc};
             if (!A->tuned) {  // the projector's first solver tunes its passes
@@ -3441,7 +3514,7 @@ void prc_solver_destroy(prc_solver* S) {
     if (!S) return;
     for (float*& b : S->raw) dfree(b);
     for (__half*& b : S->hraw) dfree(b);
-    dfree(S->sc), dfree(S->partial), dfree(S->item_begin);
+    dfree(S->sc), dfree(S->partial), dfree(S->item_begin), dfree(S->tile_begin);
     delete S;
 }
 
@@ -3452,6 +3525,7 @@ static Batch solver_batch(prc_solver* S) {
     B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd, B.queue = A->queue;
     B.fwd_shape = A->fwd_shape, B.adj_shape = A->adj_shape;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
+    B.tiles = A->tiles, B.n_tiles = A->n_tiles, B.tile_begin = S->tile_begin;
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     if (S->hraw[0]) B.gh = S->hraw[0] + S->A->margin, B.rh = S->hraw[1] + S->A->margin;
     B.sc = S->sc, B.partial = S->partial;
@@ -3526,8 +3600,8 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             b->A.assign(projectors, projectors + n);
             b->vols.resize(n);
             b->sc0.resize(n);
-            std::vector<int4> all, act, fwd;
-            std::vector<int> begin(n + 1, 0);
+            std::vector<int4> all, act, fwd, tiles;
+            std::vector<int> begin(n + 1, 0), tbegin(n + 1, 0);
             const int wbytes = projectors[0] ? projectors[0]->wbytes : 4;
             const bool accel = params[0].accelerate != 0;
             bool prior = false;
@@ -3541,12 +3615,15 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
                 b->vols[v] = A->hv;
                 b->vols[v].vbase = b->total + A->margin;  // each volume keeps its gather margins
                 begin[v] = (int)all.size();
+                tbegin[v] = (int)tiles.size();
+                for (const int4& t : A->h_tiles) tiles.push_back(make_int4(v, t.y, t.z, t.w));
                 for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
                 for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
                 for (const int4& it : A->h_items_fwd) fwd.push_back(make_int4(v, it.y, it.z, it.w));
                 b->total += A->n + 2 * (size_t)A->margin;
             }
             begin[n] = (int)all.size();
+            tbegin[n] = (int)tiles.size();
             std::vector<float> g(b->total, 0.0f);
             for (int v = 0; v < n; ++v) {
                 float vmax = 0.0f;
@@ -3567,6 +3644,11 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             HIPCHK(hipMemcpy(d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_act, act.data(), act.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_begin, begin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+            int4* d_tiles = (int4*)own(dalloc<int4>(tiles.size()));
+            int* d_tbegin = (int*)own(dalloc<int>(n + 1));
+            HIPCHK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(d_tbegin, tbegin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
+            B.tiles = d_tiles, B.n_tiles = (int)tiles.size(), B.tile_begin = d_tbegin;
             B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
             B.items_fwd = d_fwd, B.n_fwd = (int)fwd.size();
             B.queue = (int*)own(dalloc<int>(kXcds));
@@ -3583,7 +3665,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             for (int i = 0; i < 5; ++i) b->base[i] = (float*)own(dalloc<float>(b->total));
             HIPCHK(hipMemcpy(B.g, g.data(), b->total * sizeof(float), hipMemcpyHostToDevice));
             B.sc = (Scal*)own(dalloc<Scal>(n));
-            B.partial = (double*)own(dalloc<double>(2 * all.size()));
+            B.partial = (double*)own(dalloc<double>(std::max(2 * all.size(), tiles.size())));
             HIPCHK(hipStreamCreateWithFlags(&B.stream, hipStreamNonBlocking));
             HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
             HIPCHK(hipEventRecord(b->staged, B.stream));
