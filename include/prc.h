@@ -41,8 +41,14 @@ enum {
 enum {
     PRC_FLAG_EXACT_RANGE = 1,  /* --exact-range: analytic survival per step (prc.cu:75, 625) */
     PRC_FLAG_NO_CACHE_NEAR = 2, /* --no-cache-near: accepted; weights are always stored once */
-    PRC_FLAG_NO_NEAR_MEMO = 4   /* march every source's near block, even where 5x5x5 uniform
+    PRC_FLAG_NO_NEAR_MEMO = 4,  /* march every source's near block, even where 5x5x5 uniform
                                    neighbourhoods would share one (same result, for testing) */
+    PRC_FLAG_MARCH_PASSES = 8   /* store the near block only and march the far rays in every forward
+                                   and adjoint, as the build does by itself when both stored copies
+                                   would not fit in device memory (the reference retraces when its
+                                   near cache does not fit, prc.cu:411-418). Same operator; the
+                                   forward's far part is summed by atomic additions, so its last
+                                   bits may vary between runs */
 };
 
 typedef struct prc_projector prc_projector;
@@ -85,6 +91,9 @@ typedef struct {
     float cutoff_mm;         /* water-equivalent range cutoff (prc.cu:83-91, 587) */
     int64_t weight_planes;   /* source-major planes: entries of the adjoint's shift list */
     int64_t target_planes;   /* target-major planes: entries of the forward's shift list */
+    int64_t far_marched;     /* 1 when the far rays are marched in every pass (near block stored only) */
+    int64_t nonzero_entries; /* positive weights among the source-major entries (the rest are padding
+                                of lanes whose ray ended before the segment's last stored step) */
 } prc_stats;
 
 /* Library identification. */

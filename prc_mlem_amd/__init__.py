@@ -24,6 +24,7 @@ PRECISION = {"fp32": 0, "mixed": 1}
 FLAG_EXACT_RANGE = 1
 FLAG_NO_CACHE_NEAR = 2
 FLAG_NO_NEAR_MEMO = 4
+FLAG_MARCH_PASSES = 8
 RANGE_BINS = 4096
 NEAR_RAYS = 2048
 
@@ -52,7 +53,8 @@ class Stats(C.Structure):
                 ("vrs_near", C.c_int64),
                 ("vrs_far", C.c_int64), ("deposits", C.c_int64), ("near_uniform", C.c_int64),
                 ("norm_seconds", C.c_double),
-                ("cutoff_mm", C.c_float), ("weight_planes", C.c_int64), ("target_planes", C.c_int64)]
+                ("cutoff_mm", C.c_float), ("weight_planes", C.c_int64), ("target_planes", C.c_int64),
+                ("far_marched", C.c_int64), ("nonzero_entries", C.c_int64)]
 
 
 # Every symbol include/prc.h declares, with its ctypes signature.
@@ -169,7 +171,7 @@ class Projector:
     """
 
     def __init__(self, rho, spacing, isotope="68Ga", n_rays=128, support_rho=0.05, air_cap_mm=50.0,
-                 precision="fp32", exact_range=False, table=None, near_memo=True):
+                 precision="fp32", exact_range=False, table=None, near_memo=True, march_passes=False):
         rho = _volume(rho)
         if rho.ndim != 3:
             raise ValueError("rho must be a 3D (NZ, NY, NX) volume")
@@ -177,7 +179,8 @@ class Projector:
         nz, ny, nx = rho.shape
         self.geometry = Geometry(nx, ny, nz, *map(float, spacing), isotope_index(isotope), int(n_rays),
                                  float(support_rho), float(air_cap_mm), PRECISION[precision],
-                                 (FLAG_EXACT_RANGE if exact_range else 0) | (0 if near_memo else FLAG_NO_NEAR_MEMO))
+                                 (FLAG_EXACT_RANGE if exact_range else 0) | (0 if near_memo else FLAG_NO_NEAR_MEMO)
+                                 | (FLAG_MARCH_PASSES if march_passes else 0))
         self.precision = precision
         h = C.c_void_p()
         tab = None if table is None else np.ascontiguousarray(table, np.float32)

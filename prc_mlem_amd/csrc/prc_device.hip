@@ -1231,6 +1231,80 @@ __global__ __launch_bounds__(256) void k_far_fill(const Vol* __restrict__ vols, 
     }
 }
 
+// Far rays marched inside a pass, for projectors whose stored matrix does not fit in device memory
+// (only the near block is stored then; prc.cu:411-418 likewise retraces when its cache does not
+// fit). One wave per source segment marches every far ray as the fill pass does, until every lane
+// of the wave is dead, and uses each deposit w (rounded to the weight precision, as a stored weight
+// would be) at once:
+//   kMarchNorm: W_s += w (W holds the near block's sum; the same additions in the same order as
+//               the fill, so W is the stored form's bit for bit);
+//   kMarchAdj:  out_s += (sum w in[s + D]) / W_s on active s, after the stored pass wrote the near
+//               block's share;
+//   kMarchFwd:  out[s + D] += w g_s by atomic addition, after the stored pass wrote the near
+//               block's share (the order of the additions, and so the last bits, vary between runs).
+// A deposit lands only on a tissue voxel inside the grid, so every gather and every atomic stays
+// inside the volume without a margin.
+enum MarchMode { kMarchNorm = 0, kMarchAdj = 1, kMarchFwd = 2 };
+
+template <bool EXACT, class WT, class VT, int MODE>
+__global__ __launch_bounds__(256) void k_far_march(const Vol* __restrict__ vols, const int4* __restrict__ items,
+                                                    int n_items, const VT* __restrict__ vin, float* __restrict__ out) {
+    __shared__ float tab[kTabLds];
+    const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
+    const int4 it = items[min(item_id, n_items - 1)];
+    const Vol& V = vols[it.x];
+    if (!EXACT) load_table(tab, V);
+    if (item_id >= n_items || it.w <= 0) return;
+    const MarchConst M(V, tab, std::integral_constant<bool, EXACT>());
+    const int ny = V.ny, L = V.L, LF = V.LF, LP = V.LP, n_rays = V.n_rays;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int row = it.y, y = row % ny, z = row / ny, x0 = it.z;
+    const bool valid = lane < it.w;
+    const size_t loc = (size_t)row * V.nx + x0 + (valid ? lane : 0);
+    const bool act = valid && gload(V.support + loc);
+    const float wsrc = act ? gload(V.W + loc) : 0.0f;
+    float gs = 0.0f;  // the forward's f_s / W_s (already divided; 0 skips the source)
+    if (MODE == kMarchFwd) gs = act ? (float)vin[V.vbase + loc] : 0.0f;
+    // Sources that contribute nothing are dead from the start.
+    const bool live = MODE == kMarchFwd ? gs > 0.0f : MODE == kMarchAdj ? act && wsrc > 0.0f : act;
+    if (!__any(live)) return;
+    const RhoReader rho_at(V.rho_p, ((size_t)(z + 1) * (ny + 2) + (y + 1)) * (V.nx + 2) + x0 + 1 + lane);
+    const VT* vbase = vin + V.vbase;
+    float* obase = out + V.vbase;
+    float acc = MODE == kMarchNorm ? wsrc : 0.0f;
+    unsigned visits = 0;
+    for (int r = 0; r < n_rays; ++r) {
+        const int kf = gload(V.kfirst + 1 + r);
+        const FarStep* fs = V.fstep + (size_t)r * LF;
+        const int* plin = V.plin + (size_t)(1 + r) * LP;
+        Lane st = M.start(live);
+        for (int s0 = 0; s0 < L && __any(M.alive(st)); s0 += kFarGroup) {
+            float seg[kFarGroup], rv[kFarGroup];
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {  // s0 + kFarGroup - 1 < LF: zero steps past L
+                const v2i f = *(const CONST_AS v2i*)(fs + s0 + u);
+                seg[u] = __int_as_float(f[0]);
+                rv[u] = rho_at.step((unsigned)f[1]);
+            }
+#pragma unroll
+            for (int u = 0; u < kFarGroup; ++u) {
+                const int k = s0 + u;
+                const float dep = march_step<EXACT>(st, rv[u], seg[u], k >= kf, M, tab, visits);
+                const float w = WeightIO<WT>::to_float(WeightIO<WT>::store_value(dep));
+                if (MODE == kMarchNorm) {
+                    acc += w;
+                } else if (w > 0.0f) {  // k < L here: a deposit lies inside the grid
+                    const size_t t = loc + (long long)*(const CONST_AS int*)(plin + min(k, LP - 1));
+                    if (MODE == kMarchAdj) acc = fmaf(w, (float)vbase[t], acc);
+                    else unsafeAtomicAdd(obase + t, w * gs);
+                }
+            }
+        }
+    }
+    if (MODE == kMarchNorm && act) gstore(V.W + loc, acc);
+    if (MODE == kMarchAdj && act && wsrc > 0.0f) obase[loc] += acc / wsrc;
+}
+
 // Plane offsets of the segment layout, computed on the device after the counting pass: the near
 // block = 125 planes for every non-empty segment; empty segments hold nothing. The runs of a
 // segment follow each other without a gap, and every segment starts on a 64-entry boundary, so a
@@ -1242,6 +1316,25 @@ __host__ __device__ inline long long align_up(long long v, long long a) { return
 // entries follow the last list (reads of a segment's last plane group may run past its end).
 constexpr int kListAlign = 4;
 constexpr int kListSlack = 64;
+
+// Positive stored weights (nonzero bit patterns: stored weights are >= 0) among n 32-bit words
+// of a weight buffer, into *ctr.
+template <class WT>
+__global__ void k_count_nonzero(const unsigned* __restrict__ words, long long n, unsigned long long* ctr) {
+    unsigned long long c = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const unsigned v = words[i];
+        c += sizeof(WT) == 2 ? (unsigned)((v & 0xffffu) != 0) + (unsigned)((v >> 16) != 0) : (unsigned)(v != 0);
+    }
+    const long long tot = wave_sum_ll((long long)c);
+    if ((threadIdx.x & (kWave - 1)) == 0 && tot) atomicAdd(ctr, (unsigned long long)tot);
+}
+
+// Far counts of every segment cleared (the projector stores the near block only).
+__global__ void k_clear_far_counts(RowInfo* rinfo, long long units, int R1) {
+    const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < units * R1 && id % R1 != 0) rinfo[id].cnt = 0;
+}
 
 // Per segment: the runs' offsets relative to the segment, its total entries (aligned) and its
 // planes (the list length, aligned).
@@ -2269,6 +2362,9 @@ struct prc_projector {
     PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // tuned with the first solver
     bool counted = false;  // march statistics read (after the first normalisation)
     bool tuned = false;
+    // Far rays re-marched in every pass instead of stored: set by PRC_FLAG_MARCH_PASSES, or by the
+    // first normalisation when both stored copies would not fit in device memory.
+    bool far_marched = false;
     int n_act = 0, n_all = 0, n_fwd = 0;
     std::vector<int4> h_items_act, h_items_all, h_items_fwd, h_tiles;
     std::vector<int> h_row_xa, h_row_len;  // active x range of every row
@@ -2424,6 +2520,26 @@ void launch_near(prc_projector* A, hipStream_t st) {
     HIPCHK(hipGetLastError());
 }
 
+// A far-march pass (k_far_march) over the projector's active segments, described in table dv (the
+// projector's own, or a batch's, whose items then carry the batch index of the volume).
+template <class WT, class VT>
+void launch_far_march(const Vol* dv, const prc_projector* A, int mode, const int4* items, const void* in, float* out,
+                      hipStream_t st, int n_items = -1) {
+    if (!items) items = A->items_act, n_items = A->n_act;
+    const unsigned nb = blocks_for(n_items, kItemsPerBlock);
+    const VT* v = (const VT*)in;
+    ProfScope ps(mode == kMarchNorm ? "far_fill" : mode == kMarchAdj ? "adjoint_far_march" : "forward_far_march", st);
+    auto go = [&](auto exact_c) {
+        constexpr bool E = decltype(exact_c)::value;
+        if (mode == kMarchNorm) k_far_march<E, WT, VT, kMarchNorm><<<nb, 256, 0, st>>>(dv, items, n_items, v, out);
+        else if (mode == kMarchAdj) k_far_march<E, WT, VT, kMarchAdj><<<nb, 256, 0, st>>>(dv, items, n_items, v, out);
+        else k_far_march<E, WT, VT, kMarchFwd><<<nb, 256, 0, st>>>(dv, items, n_items, v, out);
+    };
+    if (A->hv.exact) go(std::true_type());
+    else go(std::false_type());
+    HIPCHK(hipGetLastError());
+}
+
 // Density upload, support, work items, range table and ray tables (everything that does not
 // depend on the march).
 void setup_projector(prc_projector* A, const float* host_rho, const float* host_table) {
@@ -2469,6 +2585,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
             }
         }
     A->stats.n_active = n_active;
+    A->far_marched = (g.flags & PRC_FLAG_MARCH_PASSES) != 0;
     if (!(g.flags & PRC_FLAG_NO_NEAR_MEMO)) {
         A->cls = dalloc<signed char>(n);
         A->near_list = dalloc<int2>(std::max<long long>(n_active, 1));
@@ -2597,6 +2714,8 @@ void ensure_list(int*& list, long long& cap, long long entries) {
 // for the forward) and W. One host synchronisation (the buffer sizes); buffers are reused.
 // Phase 1, enqueued on the projector's stream: counts, layout, and the read-back of the sizes into
 // the projector's pinned staging (normalise_finish reads them after a stream sync).
+void layout_and_sizes(prc_projector* A, hipStream_t st);
+
 void normalise_begin(prc_projector* A, hipStream_t st) {
     Vol& V = A->hv;
     const size_t units = (size_t)V.ny * V.nz * V.nseg;
@@ -2619,8 +2738,25 @@ void normalise_begin(prc_projector* A, hipStream_t st) {
             (int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
         HIPCHK(hipGetLastError());
     }
-    if (A->wbytes == 2) launch_far<__half>(A, 1, st);
-    else launch_far<float>(A, 1, st);
+    // The count pass sizes the stored layout; a projector whose far rays are marched in its passes
+    // runs it only once, for the statistics, and stores the near block alone.
+    if (!A->far_marched || !A->counted) {
+        if (A->wbytes == 2) launch_far<__half>(A, 1, st);
+        else launch_far<float>(A, 1, st);
+    }
+    layout_and_sizes(A, st);
+}
+
+// Both layouts and the read-back of their sizes into the projector's pinned staging (from the
+// counts in rinfo; with far_marched, the far counts are cleared first).
+void layout_and_sizes(prc_projector* A, hipStream_t st) {
+    Vol& V = A->hv;
+    const size_t units = (size_t)V.ny * V.nz * V.nseg;
+    const int R1 = V.n_rays + 1;
+    if (A->far_marched) {
+        k_clear_far_counts<<<blocks_for((long long)units * R1, 256), 256, 0, st>>>(A->rinfo, (long long)units, R1);
+        HIPCHK(hipGetLastError());
+    }
     {
         ProfScope ps("layout", st);
         const unsigned nu = blocks_for((long long)units, 256), nr = blocks_for((long long)units * R1, 256);
@@ -2644,10 +2780,41 @@ void normalise_begin(prc_projector* A, hipStream_t st) {
 
 // Phase 2, after phase 1 has completed: buffers, then the near block and the fill, enqueued on the
 // projector's stream.
+// Device bytes of both stored copies and their shift lists for the sizes in `total`.
+long long stored_bytes(const long long* total, int wbytes) {
+    return (total[0] + total[1] + 4 * kWave) * wbytes + (total[2] + total[3] + 2 * kListSlack) * (long long)sizeof(int);
+}
+
+// Whether the stored matrix of these sizes fits: PRC_WEIGHT_BUDGET (bytes) when set, else the free
+// device memory plus what the projector already holds, less a reserve for the solver's vectors and
+// other allocations (2 GiB or 5 % of the device, whichever is larger).
+bool stored_matrix_fits(const prc_projector* A, const long long* total) {
+    const long long need = stored_bytes(total, A->wbytes);
+    if (const char* env = getenv("PRC_WEIGHT_BUDGET")) return need <= atoll(env);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    long long held = 0;
+    if (A->wraw) held += (A->wcap + 2 * kWave) * A->wbytes;
+    if (A->traw) held += (A->tcap + 2 * kWave) * A->wbytes;
+    if (A->slist) held += (A->slcap + kListSlack) * (long long)sizeof(int);
+    if (A->tlist) held += (A->tlcap + kListSlack) * (long long)sizeof(int);
+    const long long reserve = std::max<long long>(2LL << 30, (long long)(total_b / 20));
+    return need <= (long long)free_b + held - reserve;
+}
+
 void normalise_finish(prc_projector* A, hipStream_t st) {
     Vol& V = A->hv;
     const int R1 = V.n_rays + 1;
     const long long* total = (const long long*)A->h_stage;
+    if (!A->far_marched && !stored_matrix_fits(A, total)) {
+        // The first build that does not fit switches the projector to marching its far rays in
+        // every pass; the near block alone is laid out and stored.
+        if (A->margin >= 0) fail(PRC_ERR_MEMORY, "the stored matrix no longer fits in device memory");
+        A->far_marched = true;
+        layout_and_sizes(A, st);
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    A->stats.far_marched = A->far_marched ? 1 : 0;
     const int* ctr = (const int*)(total + 4);
     const std::vector<int> kglob(ctr + 2, ctr + 2 + R1);
     if (A->cls) {
@@ -2686,10 +2853,18 @@ void normalise_finish(prc_projector* A, hipStream_t st) {
     }
     if (A->wbytes == 2) {
         launch_near<__half>(A, st);
-        launch_far<__half>(A, 2, st);
+        A->far_marched ? launch_far_march<__half, __half>(A->dv, A, kMarchNorm, nullptr, nullptr, nullptr, st)
+                       : launch_far<__half>(A, 2, st);
     } else {
         launch_near<float>(A, st);
-        launch_far<float>(A, 2, st);
+        A->far_marched ? launch_far_march<float, float>(A->dv, A, kMarchNorm, nullptr, nullptr, nullptr, st)
+                       : launch_far<float>(A, 2, st);
+    }
+    if (!A->counted) {  // statistics of the first build: the stored weights that are not zero
+        const long long words = total[0] * A->wbytes / 4;
+        if (A->wbytes == 2) k_count_nonzero<__half><<<2048, 256, 0, st>>>((const unsigned*)A->wbuf, words, A->counters + 3);
+        else k_count_nonzero<float><<<2048, 256, 0, st>>>((const unsigned*)A->wbuf, words, A->counters + 3);
+        HIPCHK(hipGetLastError());
     }
 }
 
@@ -2822,6 +2997,7 @@ void read_counters(prc_projector* A) {
     A->stats.vrs_near = (long long)c[0];
     A->stats.vrs_far = (long long)c[1];
     A->stats.deposits = (long long)c[2];
+    A->stats.nonzero_entries = (long long)c[3];
 }
 
 void release(prc_projector* A) {
@@ -2915,6 +3091,30 @@ void check_geometry(const prc_geometry* g) {
     if (g->nx + g->ny + g->nz > (1 << 24)) fail(PRC_ERR_INVALID, "grid too long for the ray tables");
 }
 
+// The far part of a projector's forward or adjoint on FP32 device vectors (far-marched projectors:
+// the stored passes above covered the near block).
+void far_march_pass(prc_projector* A, int mode, const float* in, float* out) {
+    if (A->wbytes == 2) launch_far_march<__half, float>(A->dv, A, mode, nullptr, in, out, 0);
+    else launch_far_march<float, float>(A->dv, A, mode, nullptr, in, out, 0);
+}
+
+// Far-marched volumes of a solver or batch: their active segments, tagged with the volume's index
+// in the descriptor table.
+struct MarchJob {
+    const prc_projector* A;
+    const int4* items;
+    int n;
+};
+
+void run_far_march(const Vol* dv, const std::vector<MarchJob>& jobs, int wbytes, int mode, const void* in, bool half_in,
+                   float* out, hipStream_t s) {
+    for (const MarchJob& j : jobs) {
+        if (wbytes == 4) launch_far_march<float, float>(dv, j.A, mode, j.items, in, out, s, j.n);
+        else if (half_in) launch_far_march<__half, __half>(dv, j.A, mode, j.items, in, out, s, j.n);
+        else launch_far_march<__half, float>(dv, j.A, mode, j.items, in, out, s, j.n);
+    }
+}
+
 // ---------------------------------------------------------------------------------------
 // Solver (prc.cu:659-710), all volumes of a batch at once. Buffers are concatenated voxel
 // vectors; sc (device) holds the per-volume scalars.
@@ -2933,6 +3133,7 @@ struct Batch {
     float *g = nullptr, *f = nullptr, *prev = nullptr, *y = nullptr, *hprev = nullptr, *est = nullptr,
           *gf = nullptr;
     __half *gh = nullptr, *rh = nullptr;  // mixed form: FP16 copies of the forward and adjoint inputs
+    std::vector<MarchJob> march;          // far-marched volumes (near block stored only)
     Scal* sc = nullptr;
     double* partial = nullptr;
     bool accel = false, any_prior = false;
@@ -2971,6 +3172,7 @@ void batch_iteration(const Batch& B) {
     }
     run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
                 B.fwd_shape, B.stream);
+    run_far_march(B.dv, B.march, B.wbytes, kMarchFwd, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.stream);
     {
         ProfScope ps("ratio", B.stream);
         k_ratio<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.g, B.est, pmax, B.gh != nullptr);
@@ -2981,6 +3183,7 @@ void batch_iteration(const Batch& B) {
     }
     run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev,
                 B.queue, B.adj_shape, B.stream);
+    run_far_march(B.dv, B.march, B.wbytes, kMarchAdj, B.rh ? (const void*)B.rh : B.est, B.rh != nullptr, B.prev, B.stream);
     {
         ProfScope ps("update", B.stream);
         k_update<<<B.n_tiles, 256, 0, B.stream>>>(B.dv, B.tiles, B.sc, start, B.prev, B.est, B.partial);
@@ -3354,6 +3557,7 @@ int prc_projector_forward(prc_projector* A, const float* dev_in, float* dev_out)
             HIPCHK(hipGetLastError());
         }
         run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_shape, 0);
+        if (A->far_marched) far_march_pass(A, kMarchFwd, g, dev_out);
     });
 }
 
@@ -3376,6 +3580,7 @@ int prc_projector_forward_noalloc(prc_projector* A, const float* dev_in, float* 
         k_fscale<<<blocks_for(A->n_all, kItemsPerBlock), 256>>>(A->dv, A->items_all, A->n_all, dev_in, g, nullptr);
         HIPCHK(hipGetLastError());
         run_forward(A->dv, A->items_fwd, A->n_fwd, A->wbytes, g, false, dev_out, A->queue, A->fwd_shape, 0);
+        if (A->far_marched) far_march_pass(A, kMarchFwd, g, dev_out);
     });
 }
 
@@ -3389,6 +3594,7 @@ int prc_projector_adjoint(prc_projector* A, const float* dev_in, float* dev_out)
         float* in = A->scratch_adj + A->margin;  // the input with the adjoint's gather margin
         HIPCHK(hipMemcpyAsync(in, dev_in, A->n * sizeof(float), hipMemcpyDeviceToDevice, 0));
         run_adjoint(A->dv, A->items_act, A->n_act, A->wbytes, in, false, dev_out, A->queue, A->adj_shape, 0);
+        if (A->far_marched) far_march_pass(A, kMarchAdj, in, dev_out);
     });
 }
 
@@ -3526,6 +3732,7 @@ static Batch solver_batch(prc_solver* S) {
     B.fwd_shape = A->fwd_shape, B.adj_shape = A->adj_shape;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.tiles = A->tiles, B.n_tiles = A->n_tiles, B.tile_begin = S->tile_begin;
+    if (A->far_marched) B.march.push_back(MarchJob{A, A->items_act, A->n_act});
     B.g = S->g, B.f = S->f, B.prev = S->prev, B.y = S->y, B.hprev = S->hprev, B.est = S->est, B.gf = S->gf;
     if (S->hraw[0]) B.gh = S->hraw[0] + S->A->margin, B.rh = S->hraw[1] + S->A->margin;
     B.sc = S->sc, B.partial = S->partial;
@@ -3649,6 +3856,15 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             HIPCHK(hipMemcpy(d_tiles, tiles.data(), tiles.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_tbegin, tbegin.data(), (n + 1) * sizeof(int), hipMemcpyHostToDevice));
             B.tiles = d_tiles, B.n_tiles = (int)tiles.size(), B.tile_begin = d_tbegin;
+            for (int v = 0; v < n; ++v) {  // far-marched volumes: their own active items, tagged v
+                prc_projector* A = projectors[v];
+                if (!A->far_marched) continue;
+                std::vector<int4> mine;
+                for (const int4& it : A->h_items_act) mine.push_back(make_int4(v, it.y, it.z, it.w));
+                int4* d = (int4*)own(dalloc<int4>(mine.size()));
+                HIPCHK(hipMemcpy(d, mine.data(), mine.size() * sizeof(int4), hipMemcpyHostToDevice));
+                B.march.push_back(MarchJob{A, d, (int)mine.size()});
+            }
             B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
             B.items_fwd = d_fwd, B.n_fwd = (int)fwd.size();
             B.queue = (int*)own(dalloc<int>(kXcds));

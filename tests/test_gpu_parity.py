@@ -455,3 +455,74 @@ def test_device_ray_set_matches_closed_forms_and_oracle(prc, oracle_mod):
         o = [oracle_mod.ray_f32(int(k), n) for k in ks[:200]]
         np.testing.assert_allclose(start[ks[:200]], np.array([v[0] for v in o]), rtol=0, atol=6e-8)
         np.testing.assert_allclose(direction[ks[:200]], np.array([v[1] for v in o]), rtol=0, atol=6e-8)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "mixed"])
+def test_far_marched_projector_matches_stored_and_oracle(prc, oracle_mod, pair, phantom, precision):
+    """Near block stored only, far rays marched in every pass (the memory fallback): same W bit for
+    bit as the stored form, operators and per-iteration corrections against the oracle (FP32) or the
+    stored mixed form, the reference's invariants."""
+    rho, x, y, sup = phantom
+    A, O = pair
+    announce(f"far-marched projector ({precision})")
+    S = A if precision == "fp32" else prc.Projector(rho, SPACING, "68Ga", 32, precision="mixed")
+    M = prc.Projector(rho, SPACING, "68Ga", 32, precision=precision, march_passes=True)
+    st = M.stats
+    assert st["far_marched"] == 1 and S.stats["far_marched"] == 0
+    assert st["weight_entries"] < S.stats["weight_entries"]
+    np.testing.assert_array_equal(M.W, S.W)
+    ax, aty = M.forward(x), M.adjoint(y)
+    if precision == "fp32":
+        assert rel_l2(ax, O.forward(x)) <= 1e-6
+        assert rel_l2(aty, O.adjoint(y)) <= 1e-6
+    else:
+        assert rel_l2(ax, S.forward(x)) <= 1e-6
+        assert rel_l2(aty, S.adjoint(y)) <= 1e-6
+    lhs, rhs = np.sum(ax.astype(np.float64) * y), np.sum(x.astype(np.float64) * aty)
+    assert abs(lhs / rhs - 1) <= 1e-6
+    assert abs(ax.sum(dtype=np.float64) / x.sum(dtype=np.float64) - 1) <= 1e-6
+    assert not np.any(ax[~sup])
+    if precision == "fp32":
+        _, hist = O.solve(x, 6, 0.1, 20.0, True, history=True)
+        s = prc.Solver(M, x, 0.1, 20.0, True)
+        for it, ref in enumerate(hist, 1):
+            s.iterate(1)
+            assert rel_l2(s.estimate(), ref) <= 1e-5, f"iteration {it}"
+        s.close()
+    else:
+        # The mixed solver rounds its gathered iterates to FP16 every iteration, so operator
+        # differences at float rounding (the forward's atomic order) move some of those roundings:
+        # the bar is the mixed form's own, against the stored mixed form and the FP32 oracle.
+        out = prc.correct(M, x, 6, 0.1, 20.0, True)
+        assert rel_l2(out, prc.correct(S, x, 6, 0.1, 20.0, True)) <= 1e-3
+        assert rel_l2(out, O.solve(x, 6, 0.1, 20.0, True)) <= 1e-3
+    # A batch mixing far-marched and stored volumes: each volume as its single run, to rounding.
+    outs = prc.batch_correct([M, S], [x, y], 4, 0.1, 20.0, True)
+    assert rel_l2(outs[0], prc.correct(M, x, 4, 0.1, 20.0, True)) <= (1e-6 if precision == "fp32" else 1e-4)
+    np.testing.assert_array_equal(outs[1], prc.correct(S, y, 4, 0.1, 20.0, True))
+    M.close()
+
+
+def test_build_falls_back_to_marching_when_the_matrix_does_not_fit(prc, pair, phantom, monkeypatch):
+    """A device-memory budget smaller than the stored matrix: the build switches to the far-marched
+    form by itself and gives the stored form's operator."""
+    rho, x, y, sup = phantom
+    A, _ = pair
+    monkeypatch.setenv("PRC_WEIGHT_BUDGET", "1000000")  # bytes
+    announce("build under a 1 MB weight budget")
+    M = prc.Projector(rho, SPACING, "68Ga", 32)
+    assert M.stats["far_marched"] == 1
+    np.testing.assert_array_equal(M.W, A.W)
+    assert rel_l2(M.forward(x), A.forward(x)) <= 1e-6
+    assert rel_l2(M.adjoint(y), A.adjoint(y)) <= 1e-6
+    M.normalise()  # a rebuild keeps the form
+    assert M.stats["far_marched"] == 1
+    np.testing.assert_array_equal(M.W, A.W)
+    M.close()
+
+
+def test_nonzero_entry_statistic(pair):
+    A, _ = pair
+    st = A.stats
+    assert st["deposits"] < st["nonzero_entries"] <= st["weight_entries"]
+    assert st["nonzero_entries"] <= st["deposits"] + 125 * st["n_active"]
