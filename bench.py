@@ -67,20 +67,60 @@ def measured_traffic(kernel, form, sched=None):
     return None if entry is None else entry.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(rho, data, slices, seconds_budget):
-    """The FP32 oracle (reference semantics, OpenMP) on a Z-slab of the workload: build +
-    6 iterations. Returns the measured slab time, its voxel-ray-step rate and threads."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_correction(rho, data, spacing, iso, rays, iters, beta, gamma, accel):
+    """The FP32 oracle (reference semantics) on one volume: (normalisation s, iterations s, VRS)."""
     from oracle import oracle
-    z0 = SHAPE[0] // 2 - slices // 2
-    r = np.ascontiguousarray(rho[z0:z0 + slices])
-    g = np.ascontiguousarray(data[z0:z0 + slices])
-    r = np.maximum(r, np.float32(1e-3))
+    r = np.ascontiguousarray(np.maximum(rho, np.float32(1e-3)))
+    g = np.where(r >= np.float32(0.05), np.maximum(data, 0), 0).astype(np.float32)
     t0 = time.perf_counter()
-    P = oracle.Projector(r, SPACING, ISOTOPE, RAYS)
-    P.solve(np.where(r >= np.float32(0.05), np.maximum(g, 0), 0).astype(np.float32), ITERS, BETA, GAMMA, True)
-    t = time.perf_counter() - t0
+    P = oracle.Projector(r, spacing, iso, rays)
+    t1 = time.perf_counter()
+    P.solve(g, iters, beta, gamma, accel)
+    t2 = time.perf_counter()
     near, far = P.vrs
-    return t, near + far, oracle.threads()
+    return t1 - t0, t2 - t1, near + far
+
+
+def cpu_baselines(rho, data, slices_all, slices_one, n_gpu_vrs):
+    """CPU baselines on the host, the oracle in FP32 (the reference's arithmetic; it has no CPU
+    path of its own): the 64^3 18F cylinder workload in full (Richardson-Lucy, 64 rays, 10
+    iterations), and Z-slabs of the thorax workload (the full volume would take minutes), each on
+    every host thread and on one thread. The thorax figure in corrected vols/s is the slab's
+    voxel-ray-step rate applied to the full volume's step count."""
+    from oracle import oracle
+    threads = oracle.threads()
+    out = {"cpu_model": cpu_model(), "host_threads": threads, "nproc": os.cpu_count()}
+    rho1, truth1 = phantoms.cylinder_phantom()
+    A1 = prc.Projector(rho1, (2.0, 2.0, 2.0), "18F", 64)
+    g1 = phantoms.poisson_counts(A1.forward(truth1), 1e6, seed=3001)
+    A1.close()
+    for label, nthreads, slices in (("all", threads, slices_all), ("one", 1, slices_one)):
+        oracle.set_threads(nthreads)
+        tb, ti, vrs = oracle_correction(rho1, g1, (2.0, 2.0, 2.0), "18F", 64, 10, 0.0, 20.0, False)
+        out[f"cylinder_64_18F_{label}_threads"] = {"threads": nthreads, "normalisation_s": round(tb, 3),
+                                                   "s_per_iteration": round(ti / 10, 4),
+                                                   "vols_per_s": round(1.0 / (tb + ti), 5)}
+        z0 = SHAPE[0] // 2 - slices // 2
+        tb, ti, vrs = oracle_correction(rho[z0:z0 + slices], data[z0:z0 + slices], SPACING, ISOTOPE, RAYS, ITERS,
+                                        BETA, GAMMA, True)
+        rate = vrs / (tb + ti)
+        out[f"thorax_slab_{label}_threads"] = {
+            "threads": nthreads, "slices": slices, "normalisation_s": round(tb, 3),
+            "s_per_iteration": round(ti / ITERS, 4), "vrs_per_s": round(rate, 1),
+            "full_volume_vols_per_s": round(rate / n_gpu_vrs, 6)}
+    oracle.set_threads(threads)
+    return out
 
 
 def accuracy_vs_fp64(rho, data, crop):
@@ -167,6 +207,7 @@ def main():
         fp32 = dict(vols_per_s=args.steps / t, ms_per_volume=1e3 * t / args.steps, weight_bytes=st["weight_bytes"],
                     weight_entries=st["weight_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
                     n_active=st["n_active"], deposits_far=st["deposits"], n_rows=st["n_rows"],
+                    nonzero_entries=st["nonzero_entries"],
                     target_entries=st["target_entries"], near_uniform=st["near_uniform"],
                     weight_planes=st["weight_planes"], target_planes=st["target_planes"])
         est32 = S.estimate()
@@ -222,6 +263,7 @@ def main():
                          batches_in_flight=1)
         st = As[0][0].stats
         mixed.update(weight_bytes_per_volume=st["weight_bytes"], weight_entries=st["weight_entries"],
+                     nonzero_entries=st["nonzero_entries"],
                      target_entries=st["target_entries"], vrs_near=st["vrs_near"], vrs_far=st["vrs_far"],
                      weight_planes=st["weight_planes"], target_planes=st["target_planes"])
         Bs[0].run(ITERS)
@@ -261,8 +303,13 @@ def main():
         vols = args.batch if mixed is not None else 1
         st0 = mixed if mixed is not None else fp32
         b_w = 2 if mixed is not None else 4
-        wbytes = st0["weight_entries"] * b_w * vols     # source-major planes (the adjoint's stream)
-        wbytes_t = st0["target_entries"] * b_w * vols   # target-major planes (the forward's stream)
+        # Algorithmic bytes: the nonzero stored weights (each weight of A once, in either copy),
+        # the planes' shift lists and the vectors. The zero entries that pad a segment's planes past
+        # the end of its shorter rays are streamed too but are not algorithmic; the stored-byte
+        # figures sit beside as `streamed_*`.
+        wbytes = st0["nonzero_entries"] * b_w * vols
+        wbytes_t = st0["nonzero_entries"] * b_w * vols
+        streamed = {"forward": st0["target_entries"] * b_w * vols, "adjoint": st0["weight_entries"] * b_w * vols}
         n_act = (fp32 or {}).get("n_active", int(np.sum(np.maximum(rho, 1e-3) >= 0.05)))
         b_v = 2 if mixed is not None else 4  # the mixed solver gathers FP16 copies of its iterates
         # + the planes' shift lists (4 B per plane), read once per launch like the weights
@@ -272,9 +319,12 @@ def main():
         if name in per_launch:
             mean_s = k[name]["mean_ms"] / 1e3
             achieved = per_launch[name] / mean_s / 1e9
+            streamed_bytes = per_launch[name] - (wbytes if name == "adjoint" else wbytes_t) + streamed[name]
             roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                        "bytes_per_launch": per_launch[name], "mean_ms": k[name]["mean_ms"]}
+                        "bytes_per_launch": per_launch[name], "mean_ms": k[name]["mean_ms"],
+                        "streamed_bytes_per_launch": streamed_bytes,
+                        "streamed_GBps": round(streamed_bytes / mean_s / 1e9, 1)}
         else:
             roofline = {"kernel": name, "bound": "valu", "achieved": None, "peak": None, "unit": None,
                         "frac": None, "traffic": None, "mean_ms": k[name]["mean_ms"]}
@@ -307,19 +357,21 @@ def main():
         roofline["traffic"] = measured_traffic(roofline["kernel"], form, sched)
 
     cpu = None
+    cpu_detail = None
     if args.cpu_slices > 0 and fp32 is not None:
-        log(f"cpu baseline: oracle on a {args.cpu_slices}-slice slab")
-        t_slab, vrs_slab, cores = cpu_baseline(rho, datas[0], args.cpu_slices, 30)
-        rate = vrs_slab / t_slab
-        # The oracle marches every near block: extrapolate to the full march's step count.
+        log(f"cpu baselines: oracle, all threads and one thread")
+        # The oracle marches every near block: the full volume's step count without the memo.
         full = prc.Projector(rho, SPACING, ISOTOPE, RAYS, near_memo=False)
-        vrs_near_all = full.stats["vrs_near"]
+        vrs_full = full.stats["vrs_near"] + (1 + 2 * ITERS) * fp32["vrs_far"]
         full.close()
-        vrs_full = vrs_near_all + (1 + 2 * ITERS) * fp32["vrs_far"]
-        cpu = {"value": round(rate / vrs_full, 6), "unit": "corrected vols/s", "cores": cores, "kind": "port",
-               "sample": f"FP32 oracle: normalisation + {ITERS} iterations on a {args.cpu_slices}-slice Z slab "
-                         f"({t_slab:.1f} s, {vrs_slab:.3g} voxel-ray steps); vols/s extrapolated from its "
-                         f"{rate:.3g} steps/s to the full volume's {vrs_full:.3g} steps"}
+        cpu_detail = cpu_baselines(rho, datas[0], args.cpu_slices, max(2, args.cpu_slices // 8), vrs_full)
+        slab = cpu_detail["thorax_slab_all_threads"]
+        cpu = {"value": slab["full_volume_vols_per_s"], "unit": "corrected vols/s", "cores": slab["threads"],
+               "kind": "port",
+               "sample": f"FP32 oracle, normalisation + {ITERS} iterations on a {slab['slices']}-slice Z slab of the "
+                         f"thorax workload ({slab['normalisation_s']} s + {slab['s_per_iteration']} s per iteration, "
+                         f"{slab['vrs_per_s']:.3g} voxel-ray steps/s on {slab['threads']} threads of "
+                         f"{cpu_detail['cpu_model']}); vols/s = that rate over the full volume's {vrs_full:.3g} steps"}
 
     accuracy = None
     if min(args.accuracy_crop) > 0 and args.cpu_slices > 0:
@@ -342,6 +394,7 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu, "projector_effective": projector,
     }
     out.update(result)
+    out["cpu_baselines"] = cpu_detail
     out["accuracy_vs_fp64"] = accuracy
     out["projector_tuning"] = tuning
     if fp32 is not None:

@@ -40,7 +40,8 @@ enum {
 /* Projector flags. */
 enum {
     PRC_FLAG_EXACT_RANGE = 1,  /* --exact-range: analytic survival per step (prc.cu:75, 625) */
-    PRC_FLAG_NO_CACHE_NEAR = 2, /* --no-cache-near: accepted; weights are always stored once */
+    PRC_FLAG_NO_CACHE_NEAR = 2, /* accepted and ignored (the CLI maps --no-cache-near, the reference's
+                                   memory-saving switch, to PRC_FLAG_MARCH_PASSES) */
     PRC_FLAG_NO_NEAR_MEMO = 4,  /* march every source's near block, even where 5x5x5 uniform
                                    neighbourhoods would share one (same result, for testing) */
     PRC_FLAG_MARCH_PASSES = 8   /* store the near block only and march the far rays in every forward

@@ -73,6 +73,7 @@ def lib():
         "orc_rdp_update_f32": (C.c_float, [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                            _f32p, _f32p, _u8p, C.c_long, C.c_float, C.c_float, C.c_float]),
         "orc_threads": (C.c_int, []),
+        "orc_set_threads": (None, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -262,4 +263,5 @@ def threads():
 
 
 def set_threads(n):
-    os.environ["OMP_NUM_THREADS"] = str(n)
+    """Threads of the oracle's parallel loops from now on."""
+    lib().orc_set_threads(int(n))

@@ -622,4 +622,6 @@ float orc_rdp_update_f32(int nx, int ny, int nz, float sx, float sy, float sz, c
 
 int orc_threads(void) { return omp_get_max_threads(); }
 
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 }  // extern "C"

@@ -51,7 +51,8 @@ void print_usage(const char* prog, FILE* stream = stderr) {  // prc.cu:461-484
             "  --save-every K    also write the estimate every K iterations\n"
             "  --reproject       also write A f of the result (_REPROJ.raw)\n"
             "  --exact-range     evaluate the survival function analytically, not from the table (validation)\n"
-            "  --no-cache-near   accepted for compatibility; the system matrix is always stored once\n"
+            "  --no-cache-near   store only the near block and retrace the far rays in each pass\n"
+            "                    (less device memory, same result)\n"
             "  --no-accel        plain MAP-EM without Biggs-Andrews extrapolation (validation)\n"
             "  --precision P     fp32 (default) or mixed: FP16 stored weights, FP32 arithmetic\n"
             "Isotopes:",
@@ -236,7 +237,7 @@ int main(int argc, char** argv) {
     g.nx = o.nx, g.ny = o.ny, g.nz = o.nz, g.sx = o.dx, g.sy = o.dy, g.sz = o.dz;
     g.isotope = iso, g.n_rays = o.rays, g.support_rho = o.support_rho, g.air_cap_mm = o.air_cap_mm;
     g.precision = o.precision == "mixed" ? PRC_PRECISION_MIXED : PRC_PRECISION_FP32;
-    g.flags = (o.exact_range ? PRC_FLAG_EXACT_RANGE : 0) | (o.cache_near ? 0 : PRC_FLAG_NO_CACHE_NEAR);
+    g.flags = (o.exact_range ? PRC_FLAG_EXACT_RANGE : 0) | (o.cache_near ? 0 : PRC_FLAG_MARCH_PASSES);
     double t = now_s();
     prc_projector* A = nullptr;
     check(prc_projector_create(&g, rho.data(), nullptr, &A));
