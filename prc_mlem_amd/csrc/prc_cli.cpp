@@ -71,7 +71,7 @@ void print_usage(const char* prog, FILE* stream = stderr) {
             "  --reproject       also write A f of the result (_REPROJ.raw)\n"
             "  --exact-range     evaluate the survival function analytically, not from the table (validation)\n"
             "  --no-cache-near   store only the near block and retrace the far rays in each pass\n"
-            "                    (less device memory, same result)\n"
+            "                    (less device memory, same operator)\n"
             "  --no-accel        plain MAP-EM without Biggs-Andrews extrapolation (validation)\n"
             "  --precision P     fp32 (default) or mixed: FP16 stored weights, FP32 arithmetic\n"
             "Isotopes:",

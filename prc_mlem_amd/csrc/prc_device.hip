@@ -1602,14 +1602,16 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f2 __attribute__((ext_vector_type(2)));
 
 // N consecutive values of T (float or __half) at byte offset `off` of the buffer, as floats (8, 16
-// or 32 bytes; the offset need only be aligned to T: the device reads them unaligned).
-template <class T, int N>
+// or 32 bytes; the offset need only be aligned to T: the device reads them unaligned). POL is the
+// loads' cache-policy field (1: sc0, which serves the load from L2 without allocating it in the
+// CU's L1; the tuner picks it per pass).
+template <class T, int N, int POL>
 __device__ inline void load_gather(__amdgpu_buffer_rsrc_t r, unsigned off, float (&out)[N]) {
     constexpr int bytes = N * (int)sizeof(T);
     if constexpr (bytes % 16 == 0) {
 #pragma unroll
         for (int h = 0; h < bytes / 16; ++h) {
-            const v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 16 * h), 0, 0));
+            const v4i x = __builtin_bit_cast(v4i, __builtin_amdgcn_raw_buffer_load_b128(r, (int)(off + 16 * h), 0, POL));
             if constexpr (sizeof(T) == 2) {
                 const h8 v = __builtin_bit_cast(h8, x);
 #pragma unroll
@@ -1622,7 +1624,7 @@ __device__ inline void load_gather(__amdgpu_buffer_rsrc_t r, unsigned off, float
         }
     } else {
         static_assert(bytes == 8, "loads of 8 or a multiple of 16 bytes");
-        const v2i x = __builtin_bit_cast(v2i, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+        const v2i x = __builtin_bit_cast(v2i, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, POL));
         if constexpr (sizeof(T) == 2) {
             const h4 v = __builtin_bit_cast(h4, x);
 #pragma unroll
@@ -1670,7 +1672,7 @@ __device__ inline void load_shifts(const GLOBAL_AS int* p, int (&d)[G]) {
 // group loads the next group's shifts while its own loads are in flight, so shift reads run up to
 // 2P - 1 entries past the list (into the next list or the slack: valid shifts either way).
 // Lanes with x >= len read the next plane's entries; their sums are not used.
-template <class WT, class VT>
+template <class WT, class VT, int GPOL>
 __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int planes, const int* list, const VT* vec0,
                                       int lane, float (&acc)[PlaneLanes<WT>::E]) {
     using L = PlaneLanes<WT>;
@@ -1705,7 +1707,7 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
             unsigned off = woff[g];
             if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
             load_weights<WT, E>(wrsrc, wq + off, w[g], full || q + s * G + g < planes);
-            load_gather<VT, E>(grsrc, vlane + (unsigned)d[g] * (unsigned)sizeof(VT), v[g]);
+            load_gather<VT, E, GPOL>(grsrc, vlane + (unsigned)d[g] * (unsigned)sizeof(VT), v[g]);
         }
         // Zero weights must not carry a non-finite input value into the sum (float vectors come
         // from callers of the operators). FP16 copies are finite by construction (written in
@@ -1759,21 +1761,25 @@ __device__ inline int next_item(int* queue, int n_items, int xcd, int& slot) {
 enum Sched { kSchedGrid = 0, kSchedQueue = 1, kSchedChunked = 2 };
 constexpr int kSchedCount = 3;
 
-// A pass's launch shape, chosen per solver or batch by tune_passes: the schedule (enum Sched) and,
-// for the queue schedule, its persistent grid in blocks per CU.
+// A pass's launch shape, chosen per solver or batch by tune_passes: the schedule (enum Sched), for
+// the queue schedule its persistent grid in blocks per CU, and the gathers' cache policy (0, or 1:
+// sc0, served from L2 without allocating in the CU's L1).
 struct PassShape {
     int sched = kSchedQueue;
     int blocks_per_cu = 2;
+    int gpol = 0;
 };
 const PassShape kShapes[] = {
-    {kSchedGrid, 0}, {kSchedChunked, 0}, {kSchedQueue, 1}, {kSchedQueue, 2}, {kSchedQueue, 4},
+    {kSchedGrid, 0, 0},  {kSchedChunked, 0, 0}, {kSchedQueue, 1, 0}, {kSchedQueue, 2, 0}, {kSchedQueue, 4, 0},
+    {kSchedGrid, 0, 1},  {kSchedChunked, 0, 1}, {kSchedQueue, 1, 1}, {kSchedQueue, 2, 1}, {kSchedQueue, 4, 1},
 };
 constexpr int kShapeCount = (int)(sizeof(kShapes) / sizeof(kShapes[0]));
-constexpr int kDefaultShape = 3;  // queue, two blocks per CU
+constexpr int kDefaultShape = 3;  // queue, two blocks per CU, gathers through L1
 std::string shape_name(const PassShape& p) {
     static const char* const sched[kSchedCount] = {"grid", "queue", "chunked"};
     std::string n = sched[p.sched];
     if (p.sched == kSchedQueue) n += "@" + std::to_string(p.blocks_per_cu);
+    if (p.gpol) n += "+sc0";
     return n;
 }
 
@@ -1785,7 +1791,7 @@ __device__ inline unsigned chunked_block() {
 }
 
 // out = A g where g already holds f / W (one item; see k_forward).
-template <class WT, class VT>
+template <class WT, class VT, int GPOL>
 __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const VT* __restrict__ gin,
                                     float* __restrict__ out, int lane) {
     constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
@@ -1798,7 +1804,7 @@ __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const 
     const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
     const PlaneRun r = gload(V.truns + unit);
     float acc[E];
-    segment_planes<WT, VT>((const WT*)V.tbuf + r.off, it.w, r.planes, V.tlist + r.dbase,
+    segment_planes<WT, VT, GPOL>((const WT*)V.tbuf + r.off, it.w, r.planes, V.tlist + r.dbase,
                            gin + V.vbase + (size_t)row * V.nx + x0, lane, acc);
     if (lane < LPP)
 #pragma unroll
@@ -1812,9 +1818,9 @@ __device__ __forceinline__ void forward_item(const Vol& V, const int4 it, const 
 // check; zero weights (no source) are skipped by a select where the vector is FP32 (an FP16 vector
 // is finite). Items are the active segments (w > 0)
 // and the rest of every row (w < 0, written as zeros), so every voxel of the volume is written.
-// SCHED selects the schedule (enum Sched). The gathered vector is FP32, or FP16 in the mixed
-// form (VT).
-template <class WT, class VT, int SCHED>
+// SCHED selects the schedule (enum Sched), GPOL the gathers' cache policy. The gathered vector is
+// FP32, or FP16 in the mixed form (VT).
+template <class WT, class VT, int SCHED, int GPOL>
 __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const VT* __restrict__ gin,
                                                   float* __restrict__ out, int* __restrict__ queue) {
@@ -1824,19 +1830,19 @@ __global__ __launch_bounds__(256) void k_forward(const Vol* __restrict__ vols, c
         const int i = (int)blk * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
-        if (it.w != 0) forward_item<WT, VT>(vols[it.x], it, gin, out, lane);
+        if (it.w != 0) forward_item<WT, VT, GPOL>(vols[it.x], it, gin, out, lane);
         return;
     }
     const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
-        if (it.w != 0) forward_item<WT, VT>(vols[it.x], it, gin, out, lane);
+        if (it.w != 0) forward_item<WT, VT, GPOL>(vols[it.x], it, gin, out, lane);
     }
 }
 
 // out_s = (sum_t w_st in_t) / W_s on active s (one item; see k_adjoint).
-template <class WT, class VT>
+template <class WT, class VT, int GPOL>
 __device__ __forceinline__ void adjoint_item(const Vol& V, const int4 it, const VT* __restrict__ in,
                                     float* __restrict__ out, int lane) {
     constexpr int E = PlaneLanes<WT>::E, LPP = PlaneLanes<WT>::LPP;
@@ -1844,7 +1850,7 @@ __device__ __forceinline__ void adjoint_item(const Vol& V, const int4 it, const 
     const size_t unit = (size_t)row * V.nseg + (x0 - gload(V.row_xa + row)) / kWave;
     const PlaneRun r = gload(V.sruns + unit);
     float acc[E];
-    segment_planes<WT, VT>((const WT*)V.wbuf + r.off, it.w, r.planes, V.slist + r.dbase,
+    segment_planes<WT, VT, GPOL>((const WT*)V.wbuf + r.off, it.w, r.planes, V.slist + r.dbase,
                            in + V.vbase + (size_t)row * V.nx + x0, lane, acc);
     if (lane < LPP)
 #pragma unroll
@@ -1860,7 +1866,7 @@ __device__ __forceinline__ void adjoint_item(const Vol& V, const int4 it, const 
 
 // out_s = (sum_t w_st in_t) / W_s on active s (AdjointOp, prc.cu:256-267): stream the source's
 // planes and gather in[] at s + D(p,k) (one shift per plane, margin-padded input).
-template <class WT, class VT, int SCHED>
+template <class WT, class VT, int SCHED, int GPOL>
 __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                   int n_items, const VT* __restrict__ in,
                                                   float* __restrict__ out, int* __restrict__ queue) {
@@ -1870,14 +1876,14 @@ __global__ __launch_bounds__(256) void k_adjoint(const Vol* __restrict__ vols, c
         const int i = (int)blk * kItemsPerBlock + wave_in_block();
         if (i >= n_items) return;
         const int4 it = items[i];
-        if (it.w > 0) adjoint_item<WT, VT>(vols[it.x], it, in, out, lane);
+        if (it.w > 0) adjoint_item<WT, VT, GPOL>(vols[it.x], it, in, out, lane);
         return;
     }
     const int xcd = blockIdx.x % kXcds;
     int slot = 0;
     for (int i; (i = next_item(queue, n_items, xcd, slot)) >= 0;) {
         const int4 it = items[i];
-        if (it.w > 0) adjoint_item<WT, VT>(vols[it.x], it, in, out, lane);
+        if (it.w > 0) adjoint_item<WT, VT, GPOL>(vols[it.x], it, in, out, lane);
     }
 }
 
@@ -2785,10 +2791,15 @@ long long stored_bytes(const long long* total, int wbytes) {
     return (total[0] + total[1] + 4 * kWave) * wbytes + (total[2] + total[3] + 2 * kListSlack) * (long long)sizeof(int);
 }
 
-// Whether the stored matrix of these sizes fits: PRC_WEIGHT_BUDGET (bytes) when set, else the free
-// device memory plus what the projector already holds, less a reserve for the solver's vectors and
-// other allocations (2 GiB or 5 % of the device, whichever is larger).
+// Whether the stored matrix of these sizes fits: yes when the projector's buffers already hold
+// them; else PRC_WEIGHT_BUDGET (bytes) when set, else the free device memory plus what the
+// projector already holds, less a reserve for the solver's vectors and other allocations (2 GiB or
+// 5 % of the device, whichever is larger).
 bool stored_matrix_fits(const prc_projector* A, const long long* total) {
+    // A rebuild into buffers that already hold these sizes needs no memory at all.
+    if (A->wraw && A->traw && A->slist && A->tlist && total[0] <= A->wcap && total[1] <= A->tcap &&
+        total[2] <= A->slcap && total[3] <= A->tlcap)
+        return true;
     const long long need = stored_bytes(total, A->wbytes);
     if (const char* env = getenv("PRC_WEIGHT_BUDGET")) return need <= atoll(env);
     size_t free_b = 0, total_b = 0;
@@ -3021,21 +3032,31 @@ void release(prc_projector* A) {
 // `queue` holds kXcds counters in device memory owned by the caller.
 // The gathered input is FP32, or FP16 (`half_in`, the mixed-form solver's iterates) with FP16
 // weights.
+template <class WT, class VT, int GPOL>
+void launch_forward_pol(const Vol* dv, const int4* items, int n_items, const VT* in, float* out, int* queue,
+                        const PassShape& sh, unsigned nb, hipStream_t s) {
+    if (sh.sched == kSchedQueue) k_forward<WT, VT, kSchedQueue, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else if (sh.sched == kSchedChunked) k_forward<WT, VT, kSchedChunked, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    else k_forward<WT, VT, kSchedGrid, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+}
 template <class WT, class VT>
 void launch_forward(const Vol* dv, const int4* items, int n_items, const void* g, float* out, int* queue,
                     const PassShape& sh, unsigned nb, hipStream_t s) {
-    const VT* in = (const VT*)g;
-    if (sh.sched == kSchedQueue) k_forward<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    else if (sh.sched == kSchedChunked) k_forward<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
-    else k_forward<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, in, out, queue);
+    if (sh.gpol) launch_forward_pol<WT, VT, 1>(dv, items, n_items, (const VT*)g, out, queue, sh, nb, s);
+    else launch_forward_pol<WT, VT, 0>(dv, items, n_items, (const VT*)g, out, queue, sh, nb, s);
+}
+template <class WT, class VT, int GPOL>
+void launch_adjoint_pol(const Vol* dv, const int4* items, int n_items, const VT* v, float* out, int* queue,
+                        const PassShape& sh, unsigned nb, hipStream_t s) {
+    if (sh.sched == kSchedQueue) k_adjoint<WT, VT, kSchedQueue, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    else if (sh.sched == kSchedChunked) k_adjoint<WT, VT, kSchedChunked, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    else k_adjoint<WT, VT, kSchedGrid, GPOL><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
 }
 template <class WT, class VT>
 void launch_adjoint(const Vol* dv, const int4* items, int n_items, const void* in, float* out, int* queue,
                     const PassShape& sh, unsigned nb, hipStream_t s) {
-    const VT* v = (const VT*)in;
-    if (sh.sched == kSchedQueue) k_adjoint<WT, VT, kSchedQueue><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
-    else if (sh.sched == kSchedChunked) k_adjoint<WT, VT, kSchedChunked><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
-    else k_adjoint<WT, VT, kSchedGrid><<<nb, 256, 0, s>>>(dv, items, n_items, v, out, queue);
+    if (sh.gpol) launch_adjoint_pol<WT, VT, 1>(dv, items, n_items, (const VT*)in, out, queue, sh, nb, s);
+    else launch_adjoint_pol<WT, VT, 0>(dv, items, n_items, (const VT*)in, out, queue, sh, nb, s);
 }
 
 // Persistent grid of the queue schedule: blocks_per_cu blocks (four waves each) per CU. Fewer

@@ -195,3 +195,22 @@ def test_c5_clinical_thorax_mixed_batch(prc):
     A32 = prc.Projector(rhos[0], sp, "68Ga", 256)
     ref = prc.correct(A32, datas[0], 20, 0.03, 20.0, True)
     assert rel_l2(outs[0], ref) <= 1e-3
+
+
+@pytest.mark.timeout(600)
+def test_c2_far_marched_form_matches_stored(prc, thorax):
+    """The memory fallback at the thorax's full size: W bit for bit, operators and a 6-iteration
+    correction against the stored form."""
+    rho, g, sp = thorax
+    announce("C2: far rays marched in every pass, against the stored form")
+    A = prc.Projector(rho, sp, "68Ga", 128)
+    M = prc.Projector(rho, sp, "68Ga", 128, march_passes=True)
+    assert M.stats["far_marched"] == 1
+    assert M.stats["weight_bytes"] < A.stats["weight_bytes"] / 10
+    np.testing.assert_array_equal(M.W, A.W)
+    sup = np.maximum(rho, np.float32(1e-3)) >= np.float32(0.05)
+    x = np.where(sup, g, 0).astype(np.float32)
+    assert rel_l2(M.forward(x), A.forward(x)) <= 1e-6
+    assert rel_l2(M.adjoint(x), A.adjoint(x)) <= 1e-6
+    identity_and_counts(M, sup, 36)
+    assert rel_l2(prc.correct(M, g, 6, 0.03, 20.0, True), prc.correct(A, g, 6, 0.03, 20.0, True)) <= 1e-5

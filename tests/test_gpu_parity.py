@@ -382,15 +382,15 @@ def test_concurrent_normalisation_matches_single(prc, phantom, blocks):
 @pytest.mark.parametrize("precision", ["fp32", "mixed"])
 def test_every_schedule_gives_the_same_correction(prc, phantom, blocks, precision, monkeypatch):
     # The projector passes compute each item the same way under every launch shape (launch order,
-    # XCD-local queues of 1, 2 or 4 blocks per CU, XCD-chunked launch order): the batch result must
-    # not depend on it.
+    # XCD-local queues of 1, 2 or 4 blocks per CU, XCD-chunked launch order; gathers with or without
+    # the sc0 cache policy): the batch result must not depend on it.
     rho, x, y, sup = phantom
     rho2, x2, _ = blocks
     announce(f"projector schedules ({precision})")
     A1 = prc.Projector(rho, SPACING, "68Ga", 32, precision=precision)
     A2 = prc.Projector(rho2, SPACING, "18F", 16, precision=precision)
     results = {}
-    shapes = ["grid", "chunked", "queue@1", "queue@2", "queue@4"]
+    shapes = [n + pol for pol in ("", "+sc0") for n in ("grid", "chunked", "queue@1", "queue@2", "queue@4")]
     for sched in shapes:
         monkeypatch.setenv("PRC_TUNE", sched)
         results[sched] = prc.batch_correct([A1, A2, A1], [x, x2, y], 4, beta=0.1, gamma=20.0, accelerate=True)
