@@ -214,9 +214,9 @@ int prc_profile_report(char* json, size_t capacity);
  * launch time of each launch shape of the forward and the adjoint ("grid": one wave per row
  * segment in launch order; "chunked": the same with each XCD's blocks renumbered to one contiguous
  * chunk of the segments; "queue@B": a persistent grid of B = 1, 2 or 4 blocks per CU taking
- * segments from XCD-local work queues) and the one kept. No counterpart in the reference (prc.cu
- * has one fixed launch shape); PRC_TUNE=0 disables tuning (queue@2), PRC_TUNE=<shape> forces one
- * ("queue" alone is queue@2). */
+ * segments from XCD-local work queues; "+sc0": gathers served from L2 without allocating in the
+ * CU's L1) and the one kept. No counterpart in the reference (prc.cu has one fixed launch shape);
+ * PRC_TUNE=0 disables tuning (queue@2), PRC_TUNE=<shape> forces one ("queue" alone is queue@2). */
 int prc_tuning_report(char* json, size_t capacity);
 
 #ifdef __cplusplus
