@@ -997,15 +997,35 @@ __global__ __launch_bounds__(kMemoBlock) void k_memo_list(int n, const signed ch
 // One wave per segment; step lengths and shifts come from scalar loads (the step table is read
 // through the constant address space: it is not written while kernels run). COUNT adds the
 // number of voxel visits of live lanes to *vrs.
+// Zeroes store instructions [i0, i1) of a wave's share of a clear range: instruction i writes the
+// 64 16-byte units base + 64 i + lane that lie below `units`.
+__device__ inline void clear_units(v4i* __restrict__ clear, long long units, long long base, long long i0,
+                                   long long i1, int lane) {
+    for (long long i = i0; i < i1; ++i) {
+        const long long e = base + i * kWave + lane;
+        if (e < units) *(GLOBAL_AS v4i*)(clear + e) = v4i{0, 0, 0, 0};
+    }
+}
+
+// Far rays, count pass. `clear` (units 16-byte units, or none) is a buffer this pass zeroes beside
+// the march: a rebuild clears the target-major copy it is about to refill here, where the march
+// leaves the memory bus idle, instead of in a memory fill of its own after the layout. Each wave
+// zeroes an equal share, spread over its rays (waves without a segment zero theirs at once).
 template <bool COUNT>
 __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols, const int4* __restrict__ items,
                                                     int n_items, const FarStep* __restrict__ fstep,
                                                     const int* __restrict__ kfirst, RowInfo* __restrict__ rinfo,
-                                                    unsigned long long* vrs) {
+                                                    unsigned long long* vrs, v4i* __restrict__ clear, long long units) {
     const int item_id = blockIdx.x * kItemsPerBlock + wave_in_block();
     if (item_id >= n_items) return;
     const int4 it = items[item_id];
-    if (it.w <= 0) return;
+    const long long waves = (long long)gridDim.x * kItemsPerBlock;
+    const long long share = (units + waves * kWave - 1) / (waves * kWave);  // store instructions per wave
+    const long long cbase = (long long)item_id * share * kWave;
+    if (it.w <= 0) {
+        clear_units(clear, units, cbase, 0, share, threadIdx.x & (kWave - 1));
+        return;
+    }
     const Vol& V = vols[it.x];
     const int ny = V.ny, n_rays = V.n_rays, R1 = n_rays + 1, LF = V.LF;
     const float cutoff = V.cutoff, support_rho = V.support_rho, air_cap = V.air_cap;
@@ -1025,6 +1045,7 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
     const int* box = V.box;
     const int xl = x0 + lane;
     for (int r = 0; r < n_rays; ++r) {
+        clear_units(clear, units, cbase, share * r / n_rays, share * (r + 1) / n_rays, lane);
         const int kf = kfirst[1 + r];
         const FarStep* fs = fstep + (size_t)r * LF;
         float w = act ? 0.0f : cutoff, air = 0.0f;
@@ -2367,6 +2388,7 @@ struct prc_projector {
     void* h_stage = nullptr;
     PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // tuned with the first solver
     bool counted = false;  // march statistics read (after the first normalisation)
+    bool tcleared = false;  // the count pass of the current normalisation zeroed traw (all of tcap)
     bool tuned = false;
     // Far rays re-marched in every pass instead of stored: set by PRC_FLAG_MARCH_PASSES, or by the
     // first normalisation when both stored copies would not fit in device memory.
@@ -2479,12 +2501,18 @@ void launch_far(prc_projector* A, int pass, hipStream_t st) {
     const unsigned nb = blocks_for(A->n_act, kItemsPerBlock);
     if (pass == 1) {
         ProfScope ps("far_count", st);
+        // A rebuild zeroes the target-major copy it will refill during the count (the fill writes
+        // only the entries that receive a deposit); normalise_finish clears it itself when the
+        // buffer is new or grows.
+        A->tcleared = A->traw != nullptr && !A->far_marched;
+        v4i* clear = A->tcleared ? (v4i*)A->traw : nullptr;
+        const long long units = A->tcleared ? (A->tcap + 2 * kWave) * A->wbytes / 16 : 0;
         if (count)
             k_far_count<true><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, A->fstep, A->kfirst, A->rinfo,
-                                                  A->counters + 1);
+                                                  A->counters + 1, clear, units);
         else
             k_far_count<false><<<nb, 256, 0, st>>>(A->dv, A->items_act, A->n_act, A->fstep, A->kfirst, A->rinfo,
-                                                   nullptr);
+                                                   nullptr, clear, units);
     } else {
         ProfScope ps("far_fill", st);
         unsigned long long* ndep = A->counters + 2;
@@ -2841,7 +2869,10 @@ void normalise_finish(prc_projector* A, hipStream_t st) {
     if (A->margin < 0) A->margin = margin;
     else if (margin > A->margin) fail(PRC_ERR_INVALID, "the system matrix reach grew after the first normalisation");
     ensure_weights(A->wraw, A->wbuf, A->wcap, total[0], A->wbytes);
+    const long long tcap_before = A->tcap;  // ensure_weights changes the capacity when it reallocates
     ensure_weights(A->traw, A->tbuf, A->tcap, total[1], A->wbytes);
+    const bool tzero = A->tcleared && A->tcap == tcap_before;  // zeroed by the count pass, buffer kept
+    A->tcleared = false;
     if (total[2] > INT_MAX - kListSlack || total[3] > INT_MAX - kListSlack)
         fail(PRC_ERR_INVALID, "the system matrix holds too many planes for 32-bit shift-list offsets");
     ensure_list(A->slist, A->slcap, total[2]);
@@ -2854,7 +2885,7 @@ void normalise_finish(prc_projector* A, hipStream_t st) {
     V.wbuf = A->wbuf, V.tbuf = A->tbuf, V.slist = A->slist, V.tlist = A->tlist;
     HIPCHK(hipMemcpyAsync(A->dv, &V, sizeof(Vol), hipMemcpyHostToDevice, st));
     // The fill writes only the target-major entries that receive a deposit: the rest must be zero.
-    HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
+    if (!tzero) HIPCHK(hipMemsetAsync(A->traw, 0, (size_t)(A->tcap + 2 * kWave) * A->wbytes, st));
     {
         const size_t units = (size_t)V.ny * V.nz * V.nseg;
         const unsigned nb = blocks_for((long long)units, 256 / kWave);
