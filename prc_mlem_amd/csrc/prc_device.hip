@@ -1335,8 +1335,14 @@ __host__ __device__ inline long long run_size(int cnt, int w) { return (long lon
 __host__ __device__ inline long long align_up(long long v, long long a) { return (v + a - 1) / a * a; }
 // Shift lists: each segment's list starts on a multiple of kListAlign entries, and kListSlack zero
 // entries follow the last list (reads of a segment's last plane group may run past its end).
-constexpr int kListAlign = 4;
+// With FP16 weights the shifts of every aligned block of 8 planes are stored permuted (list_slot):
+// the projector passes read plane pairs, and a lane's four planes 2g + h (g = 0..3) of a block
+// must be four consecutive entries.
+constexpr int kListAlign = 8;
 constexpr int kListSlack = 64;
+__host__ __device__ inline long long list_slot(long long i, bool pairs) {
+    return pairs ? (i & ~7LL) | ((i & 1) << 2) | ((i >> 1) & 3) : i;
+}
 
 // Positive stored weights (nonzero bit patterns: stored weights are >= 0) among n 32-bit words
 // of a weight buffer, into *ctr.
@@ -1553,7 +1559,7 @@ __global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base,
 // in plane order; the alignment padding of each list is zero. One wave per segment: the wave
 // reads the runs' counts 64 rays at a time and places each run by a prefix sum over its lanes.
 __global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __restrict__ info,
-                                                      const PlaneRun* __restrict__ runs, int sign,
+                                                      const PlaneRun* __restrict__ runs, int sign, int pairs,
                                                       int* __restrict__ list) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
     const int unit = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
@@ -1578,11 +1584,11 @@ __global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __res
         for (int j = 0; j < n; ++j) {  // ray p0 + j: its count and place from lane j
             const int c = __shfl(cnt, j), f = __shfl(first, j), k0 = __shfl(kf, j);
             const int* src = V.plin + (size_t)(p0 + j) * V.LP + k0;
-            for (int k = lane; k < c; k += kWave) out[f + k] = sign * src[k];
+            for (int k = lane; k < c; k += kWave) out[list_slot(f + k, pairs)] = sign * src[k];
         }
         pos += __shfl(incl, kWave - 1);
     }
-    if (lane < align_up(pos, kListAlign) - pos) out[pos + lane] = 0;
+    if (lane < align_up(pos, kListAlign) - pos) out[list_slot(pos + lane, pairs)] = 0;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1597,9 +1603,14 @@ __device__ inline float forward_scale(float f, float w) {
 // Plane-parallel streaming. A wave streams the plane sequence of one segment. Each lane loads 16
 // bytes of a plane: E = 16 / sizeof(WT) consecutive entries (x = c*E .. c*E + E - 1 with
 // c = lane % LPP), so one load instruction covers PPI = E planes, and a group of G loads covers
-// P = PPI * G = 32 planes. Lane sub-plane s = lane / LPP takes planes q + s*G + g (g < G) of the
-// group starting at plane q, so its G shifts are consecutive list entries (16-byte loads).
-// Each lane keeps E running sums; at the end the PPI lanes of each x chunk are added together.
+// P = PPI * G = 32 planes. With FP32 weights lane sub-plane s = lane / LPP takes planes
+// q + s*G + g (g < G) of the group starting at plane q: each load reads 4 planes of 256 bytes.
+// With FP16 weights (128-byte planes) load g reads the plane pairs 8j + 2g, 8j + 2g + 1 (j < 4),
+// which lie next to each other: 4 runs of 256 bytes, which the texture units read in about 3/4 of
+// the time of 8 separate planes; lane sub-plane s = 2j + h takes planes 8j + 2g + h. Either way a
+// lane's G shifts are consecutive list entries (16-byte loads; FP16 lists are stored permuted
+// within blocks of 8, see list_slot). Each lane keeps E running sums; at the end the PPI lanes of
+// each x chunk are added together.
 template <class WT>
 struct PlaneLanes {
     static constexpr int E = 16 / (int)sizeof(WT);  // entries per lane and load
@@ -1607,7 +1618,8 @@ struct PlaneLanes {
     static constexpr int PPI = E;                   // planes per load instruction
     static constexpr int G = 32 / PPI;              // loads per group
     static constexpr int P = PPI * G;               // planes per group
-    static_assert(G % kListAlign == 0, "a lane's shifts load as int4s from an aligned list");
+    static_assert(G == 4 || G == 8, "a lane's shifts load as int4s");
+    static_assert(kListAlign % 8 == 0, "lists start on int4 boundaries and hold whole 8-plane blocks");
     static_assert(2 * P <= kListSlack, "the shift prefetch of a last group reads up to 2P - 1 past a list");
 };
 
@@ -1707,9 +1719,11 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
     const __amdgpu_buffer_rsrc_t wrsrc = buffer_of(wseg, 0x7fffffffLL);
     unsigned wq = 0;  // byte offset of the current group's first plane
     const unsigned group_bytes = (unsigned)(P * len * (int)sizeof(WT));
+    // Plane of the group read by this lane's load g (see above).
+    auto plane_of = [&](int g) { return sizeof(WT) == 2 ? 8 * (s >> 1) + 2 * g + (s & 1) : s * G + g; };
     unsigned woff[G];
 #pragma unroll
-    for (int g = 0; g < G; ++g) woff[g] = (unsigned)(((s * G + g) * len + c * E) * (int)sizeof(WT));
+    for (int g = 0; g < G; ++g) woff[g] = (unsigned)((plane_of(g) * len + c * E) * (int)sizeof(WT));
     // Gathers through a buffer descriptor whose base is moved back by kGatherBias bytes (only
     // base + offset is ever dereferenced, always inside the vector's allocation).
     const __amdgpu_buffer_rsrc_t grsrc =
@@ -1726,8 +1740,8 @@ __device__ __forceinline__ void segment_planes(const WT* wseg, int len, int plan
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             unsigned off = woff[g];
-            if (!full) off = (unsigned)((min(s * G + g, planes - 1 - q) * len + c * E) * (int)sizeof(WT));
-            load_weights<WT, E>(wrsrc, wq + off, w[g], full || q + s * G + g < planes);
+            if (!full) off = (unsigned)((min(plane_of(g), planes - 1 - q) * len + c * E) * (int)sizeof(WT));
+            load_weights<WT, E>(wrsrc, wq + off, w[g], full || q + plane_of(g) < planes);
             load_gather<VT, E, GPOL>(grsrc, vlane + (unsigned)d[g] * (unsigned)sizeof(VT), v[g]);
         }
         // Zero weights must not carry a non-finite input value into the sum (float vectors come
@@ -2889,8 +2903,9 @@ void normalise_finish(prc_projector* A, hipStream_t st) {
     {
         const size_t units = (size_t)V.ny * V.nz * V.nseg;
         const unsigned nb = blocks_for((long long)units, 256 / kWave);
-        k_plane_lists<<<nb, 256, 0, st>>>(V, A->rinfo, A->sruns, 1, A->slist);
-        k_plane_lists<<<nb, 256, 0, st>>>(V, A->tinfo, A->truns, -1, A->tlist);
+        const int pairs = A->wbytes == 2;
+        k_plane_lists<<<nb, 256, 0, st>>>(V, A->rinfo, A->sruns, 1, pairs, A->slist);
+        k_plane_lists<<<nb, 256, 0, st>>>(V, A->tinfo, A->truns, -1, pairs, A->tlist);
         HIPCHK(hipGetLastError());
     }
     if (A->wbytes == 2) {
@@ -2983,7 +2998,7 @@ void audit_layout(prc_projector* A) {
                     const long long d = pl[(size_t)p * LP + kf[p] + k];
                     if (d > shift_max || d < -shift_max)
                         fail(PRC_ERR_INVALID, "audit: %s shift %lld of ray %d exceeds the margin", what, d, p);
-                    if (list[(size_t)(lnext + planes + k)] != sign * d)
+                    if (list[(size_t)list_slot(lnext + planes + k, A->wbytes == 2)] != sign * d)
                         fail(PRC_ERR_INVALID, "audit: %s shift list of segment %zu ray %d step %d is wrong", what, u, p, k);
                 }
                 off += run_size(e.cnt, e.len);
@@ -2991,7 +3006,8 @@ void audit_layout(prc_projector* A) {
             }
             if (r.planes != planes) fail(PRC_ERR_INVALID, "audit: %s segment %zu counts %d planes, not %d", what, u, r.planes, planes);
             for (long long k = planes; k < align_up(planes, kListAlign); ++k)
-                if (list[(size_t)(lnext + k)] != 0) fail(PRC_ERR_INVALID, "audit: %s list padding is not zero", what);
+                if (list[(size_t)list_slot(lnext + k, A->wbytes == 2)] != 0)
+                    fail(PRC_ERR_INVALID, "audit: %s list padding is not zero", what);
             // Plane reads: lanes read up to one wave past the last plane's start; shift reads (the
             // last group's prefetch) up to 2P - 2 entries past the list's end.
             if (planes > 0 && (r.off < 0 || r.off + (long long)(planes - 1) * w + kWave - 1 >= total + kWave))
