@@ -1363,30 +1363,66 @@ __global__ void k_clear_far_counts(RowInfo* rinfo, long long units, int R1) {
     if (id < units * R1 && id % R1 != 0) rinfo[id].cnt = 0;
 }
 
+// Inclusive prefix sum of v over the wave's lanes.
+__device__ inline int wave_incl_sum(int v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int x = __shfl_up(v, o);
+        if (lane >= o) v += x;
+    }
+    return v;
+}
+
+// Runs of one segment from their counts (cnt[p], lane = ray within a chunk of 64): each run's
+// offset relative to the segment and its record; returns the segment's entries and planes. The
+// runs of a segment all have its width, so the offsets are the width times a prefix sum of the
+// counts. One wave per segment.
+template <class CountOf>
+__device__ inline void layout_runs(const Vol& V, RowInfo* info, int unit, int w, int x0, CountOf&& count_of,
+                                   long long& entries, int& planes) {
+    const int R1 = V.n_rays + 1, lane = threadIdx.x & (kWave - 1);
+    entries = 0, planes = 0;
+    for (int p0 = 0; p0 < R1; p0 += kWave) {
+        const int p = p0 + lane;
+        const int cnt = p < R1 && w > 0 ? count_of(p) : 0;
+        const int incl = wave_incl_sum(cnt);
+        if (p < R1) {
+            RowInfo* e = info + (size_t)unit * R1 + p;
+            e->cnt = cnt;
+            e->off = entries + (long long)(incl - cnt) * w;  // run_size(cnt, w) = cnt * w
+            e->xa = x0;
+            e->len = w;
+        }
+        const int tot = __shfl(incl, kWave - 1);
+        entries += (long long)tot * w;
+        planes += tot;
+    }
+}
+
 // Per segment: the runs' offsets relative to the segment, its total entries (aligned) and its
-// planes (the list length, aligned).
-__global__ void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs, int* cnt_t) {
+// planes (the list length, aligned). One wave per segment, lanes over rays.
+__global__ __launch_bounds__(256) void k_row_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs,
+                                                     int* cnt_t) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
-    const int unit = blockIdx.x * blockDim.x + threadIdx.x;
+    const int unit = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
     if (unit >= units) return;
     const int row = unit / V.nseg, seg = unit % V.nseg;
     const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
-    long long off = 0;
-    int planes = 0;
-    for (int p = 0; p < R1; ++p) {
-        RowInfo* e = V.rinfo + (size_t)unit * R1 + p;
-        const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : e->cnt);
-        e->cnt = cnt;
+    long long entries;
+    int planes;
+    layout_runs(V, V.rinfo, unit, w, x0, [&](int p) {
+        const int cnt = p == 0 ? kNQ : V.rinfo[(size_t)unit * R1 + p].cnt;
         cnt_t[(size_t)p * units + unit] = cnt;  // ray-major copy of the counts
-        e->off = off;
-        e->xa = x0;
-        e->len = w;
-        off += run_size(cnt, w);
-        planes += cnt;
+        return cnt;
+    }, entries, planes);
+    if (w == 0)
+        for (int p = threadIdx.x & (kWave - 1); p < R1; p += kWave) cnt_t[(size_t)p * units + unit] = 0;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unit_tot[unit] = align_up(entries, kWave);
+        unit_planes[unit] = align_up(planes, kListAlign);
+        runs[unit].planes = planes;
     }
-    unit_tot[unit] = align_up(off, kWave);
-    unit_planes[unit] = align_up(planes, kListAlign);
-    runs[unit].planes = planes;
 }
 
 // Exclusive prefix sums of the per-segment entries and list lengths, in three launches: the sums of
@@ -1517,29 +1553,25 @@ __global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int
 // Target-major layout: segment t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
 // source deposits into it (the near block: all 125 bins); planes over the segment's x range, runs
 // without gaps as in the source-major copy.
-__global__ void k_target_totals(Vol V, long long* unit_tot, long long* unit_planes, PlaneRun* runs) {
+__global__ __launch_bounds__(256) void k_target_totals(Vol V, long long* unit_tot, long long* unit_planes,
+                                                        PlaneRun* runs) {
     const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
-    const int unit = blockIdx.x * blockDim.x + threadIdx.x;
+    const int unit = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
     if (unit >= units) return;
     const int row = unit / V.nseg, seg = unit % V.nseg;
     const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
-    long long off = 0;
-    int planes = 0;
-    for (int p = 0; p < R1; ++p) {
-        const int kt = V.rinfo[(size_t)unit * R1 + p].kt;
-        const int cnt = w == 0 ? 0 : (p == 0 ? kNQ : max(0, kt - gload(V.kfirst + p)));
-        RowInfo* e = V.tinfo + (size_t)unit * R1 + p;
-        e->cnt = cnt;
-        e->kt = kt;
-        e->off = off;
-        e->xa = x0;
-        e->len = w;
-        off += run_size(cnt, w);
-        planes += cnt;
+    for (int p = threadIdx.x & (kWave - 1); p < R1; p += kWave)
+        V.tinfo[(size_t)unit * R1 + p].kt = V.rinfo[(size_t)unit * R1 + p].kt;
+    long long entries;
+    int planes;
+    layout_runs(V, V.tinfo, unit, w, x0, [&](int p) {
+        return p == 0 ? kNQ : max(0, V.rinfo[(size_t)unit * R1 + p].kt - gload(V.kfirst + p));
+    }, entries, planes);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        unit_tot[unit] = align_up(entries, kWave);
+        unit_planes[unit] = align_up(planes, kListAlign);
+        runs[unit].planes = planes;
     }
-    unit_tot[unit] = align_up(off, kWave);
-    unit_planes[unit] = align_up(planes, kListAlign);
-    runs[unit].planes = planes;
 }
 
 __global__ void k_target_offsets(Vol V, const long long* __restrict__ unit_base,
@@ -1579,14 +1611,25 @@ __global__ __launch_bounds__(256) void k_plane_lists(Vol V, const RowInfo* __res
             const int x = __shfl_up(incl, o);
             if (lane >= o) incl += x;
         }
-        const int first = pos + incl - cnt;
-        const int n = min(kWave, R1 - p0);
-        for (int j = 0; j < n; ++j) {  // ray p0 + j: its count and place from lane j
-            const int c = __shfl(cnt, j), f = __shfl(first, j), k0 = __shfl(kf, j);
-            const int* src = V.plin + (size_t)(p0 + j) * V.LP + k0;
-            for (int k = lane; k < c; k += kWave) out[list_slot(f + k, pairs)] = sign * src[k];
+        const int frel = incl - cnt;  // the lane's run starts at plane frel of the chunk
+        const int tot = __shfl(incl, kWave - 1);
+        // Lanes take the chunk's planes in stripes of 64 (coalesced stores); the run holding plane q
+        // is the last whose start is at most q (runs of zero planes share the next run's start and
+        // lose to it), found by a binary search over the lanes' starts.
+        // (The loop runs the same number of times on every lane, so that every shuffle reads
+        // lanes that take part in it.)
+        for (int q0 = 0; q0 < tot; q0 += kWave) {
+            const int q = q0 + lane;
+            int j = 0, fj = 0;
+#pragma unroll
+            for (int step = kWave / 2; step >= 1; step >>= 1) {
+                const int fc = __shfl(frel, j + step);
+                if (fc <= q) j += step, fj = fc;
+            }
+            const int k0 = __shfl(kf, j);
+            if (q < tot) out[list_slot(pos + q, pairs)] = sign * gload(V.plin + (size_t)(p0 + j) * V.LP + k0 + (q - fj));
         }
-        pos += __shfl(incl, kWave - 1);
+        pos += tot;
     }
     if (lane < align_up(pos, kListAlign) - pos) out[list_slot(pos + lane, pairs)] = 0;
 }
@@ -2807,13 +2850,14 @@ void layout_and_sizes(prc_projector* A, hipStream_t st) {
     }
     {
         ProfScope ps("layout", st);
-        const unsigned nu = blocks_for((long long)units, 256), nr = blocks_for((long long)units * R1, 256);
-        k_row_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
+        const unsigned nr = blocks_for((long long)units * R1, 256);
+        const unsigned nw = blocks_for((long long)units, 256 / kWave);  // one wave per segment
+        k_row_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
         scan_units(A, A->total, A->total + 2, st);
         k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
         k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
         k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t);
-        k_target_totals<<<nu, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
+        k_target_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
         scan_units(A, A->total + 1, A->total + 3, st);
         k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns, A->tmaj);
         HIPCHK(hipGetLastError());
