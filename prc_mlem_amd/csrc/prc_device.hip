@@ -1530,21 +1530,38 @@ __global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int
     if (tw > 0 && p > 0) {  // the near block (p = 0) always holds its 125 bins
         const int tx0 = gload(V.row_xa + row) + kWave * seg, y = row % V.ny, z = row / V.ny;
         const int kf = gload(V.kfirst + p), kend = gload(kglob + p);
+        const int* cnt_p = cnt_t + (size_t)p * units;
         // Downward from the ray's global bound: the first step that reaches this segment is the bound.
-        for (int k = kend - 1; k >= kf && kt == 0; --k) {
-            const int4 d = gload(V.pstep + (size_t)p * V.LP + k);
-            const int ys = y - d.y, zs = z - d.z;
-            if ((unsigned)ys >= (unsigned)V.ny || (unsigned)zs >= (unsigned)V.nz) continue;
-            const int srow = zs * V.ny + ys;
-            const int slen = gload(V.row_len + srow), sxa = gload(V.row_xa + srow);
-            for (int ss = 0; ss * kWave < slen; ++ss) {
-                const int sx0 = sxa + kWave * ss, sw = segment_width(slen, ss);
-                if (sx0 + d.x < tx0 + tw && tx0 < sx0 + sw + d.x &&
-                    k - kf < cnt_t[(size_t)p * units + srow * nseg + ss]) {
-                    kt = k + 1;
-                    break;
+        // Four steps at a time, so that their dependent loads (step, source row, counts) overlap.
+        // The sources that reach x range [tx0, tx0 + tw) of this row at step k lie in x range
+        // [tx0 - dx, tx0 - dx + tw) of row (y - dy, z - dz): at most two of its segments.
+        constexpr int U = 4;
+        for (int k = kend - 1; k >= kf && kt == 0; k -= U) {
+            int4 d[U];
+            int srow[U], slen[U], sxa[U], c0[U], c1[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) d[u] = k - u >= kf ? gload(V.pstep + (size_t)p * V.LP + k - u) : int4{0, 0, 0, 0};
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int ys = y - d[u].y, zs = z - d[u].z;
+                const bool in = k - u >= kf && (unsigned)ys < (unsigned)V.ny && (unsigned)zs < (unsigned)V.nz;
+                srow[u] = in ? zs * V.ny + ys : -1;
+                slen[u] = in ? gload(V.row_len + srow[u]) : 0;
+                sxa[u] = in ? gload(V.row_xa + srow[u]) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int lo = max(tx0 - d[u].x, sxa[u]), hi = min(tx0 - d[u].x + tw, sxa[u] + slen[u]);
+                c0[u] = c1[u] = 0;
+                if (lo < hi) {
+                    const int s0 = (lo - sxa[u]) / kWave, s1 = (hi - 1 - sxa[u]) / kWave;
+                    c0[u] = gload(cnt_p + (size_t)srow[u] * nseg + s0);
+                    if (s1 != s0) c1[u] = gload(cnt_p + (size_t)srow[u] * nseg + s1);
                 }
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (kt == 0 && (k - u - kf < c0[u] || k - u - kf < c1[u])) kt = k - u + 1;
         }
     }
     V.rinfo[(size_t)unit * R1 + p].kt = kt;
