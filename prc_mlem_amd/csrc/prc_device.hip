@@ -46,6 +46,11 @@ static_assert(kNearRays % 2 == 0, "the near march takes rays in pairs");
 constexpr int kNQ = 125;                 // 5x5x5 near block (prc.cu:103)
 constexpr int kNearMax = 8;              // a near ray visits at most 7 block voxels
 constexpr float kMinDensity = 1.0e-3f;   // prc.cu:102
+// Density outside the grid (the padded volume's halo and margins): a quiet NaN. No comparison
+// holds for it, so such a voxel is never tissue and never counts as a visit, and
+// fminf(fmaf(NaN, seg, w), cutoff) = cutoff (fminf returns its non-NaN operand), so a step onto
+// it leaves the lane dead without a deposit: the grid exit of prc.cu:163.
+constexpr float kHalo = __builtin_nanf("");
 constexpr int kWave = 64;
 constexpr int kItemsPerBlock = 4;        // 256-thread blocks, one 64-voxel row segment per wave
 constexpr int kFarGroup = 8;             // far-march steps whose density loads are issued together
@@ -185,8 +190,8 @@ struct Vol {            // device descriptor of one volume
     int LF;             // step-table stride: L + kFarGroup rounded up to kMarchGroup, zero steps past L
     Branch br[3];
     size_t vbase;       // offset of this volume in concatenated voxel vectors
-    const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, +inf halo
-    const float* rho_base;  // start of the allocation holding rho_p and its +inf margins
+    const float* rho_p; // (nx+2)(ny+2)(nz+2), clamped density, kHalo (NaN) halo
+    const float* rho_base;  // start of the allocation holding rho_p and its kHalo margins
     long long rho_bytes;    // size of that allocation
     const float* table; // kBins + 1
     const unsigned char* support;
@@ -492,23 +497,23 @@ struct MarchConst {
 };
 
 // One DDA step of prc.cu:146-163 for the current lane; returns the deposit (0 if none). Written
-// with selects instead of branches: the ray leaves the grid at the +inf halo, dies in air after
+// with selects instead of branches: the ray leaves the grid at the kHalo halo, dies in air after
 // air_cap mm, and dies when the range reaches the cutoff. A dead lane (w = cutoff) computes
 // wn = cutoff and sn = S(cutoff) = surv, so it deposits nothing and stays dead.
 template <bool EXACT>
 __device__ inline float march_step(Lane& st, float rho, float seg, bool deposit_here, const MarchConst& V,
                                    const float* tab, unsigned& visits) {
-    const bool inside = rho < INFINITY;
-    visits += (inside && st.w < V.cutoff) ? 1u : 0u;
+    visits += (rho < INFINITY && st.w < V.cutoff) ? 1u : 0u;  // a voxel of the grid (not kHalo)
     const float wn = fminf(fmaf(rho, seg, st.w), V.cutoff);
     const float sn = EXACT ? survival(V.br, V.nb, wn) : march_lookup(tab, V.scale, wn);
     const bool tissue = rho >= V.support_rho;
     const float air = tissue ? 0.0f : st.air + seg;
     // surv - sn > 0 exactly when surv > sn, so the max is the reference's guarded difference.
-    const float dep = (inside && tissue && deposit_here) ? fmaxf(st.surv - sn, 0.0f) : 0.0f;
-    // In tissue air = 0 and air_cap > 0, so the air test needs no tissue test. A step onto the +inf
-    // halo needs no test either: wn = fminf(fmaf(inf, seg, w), cutoff) = cutoff (for seg = 0 too,
-    // fminf returning the non-NaN operand), so sn = S(cutoff) = s_cut, the state of a dead lane.
+    // A kHalo voxel is not tissue, so it deposits nothing.
+    const float dep = (tissue && deposit_here) ? fmaxf(st.surv - sn, 0.0f) : 0.0f;
+    // In tissue air = 0 and air_cap > 0, so the air test needs no tissue test. A step onto the
+    // halo needs no test either: wn = fminf(fmaf(NaN, seg, w), cutoff) = cutoff, so
+    // sn = S(cutoff) = s_cut, the state of a dead lane.
     const bool dies = air > V.air_cap;
     st.w = dies ? V.cutoff : wn;
     st.surv = dies ? V.s_cut : sn;
@@ -612,7 +617,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
     // ray order (a ray visits a voxel once), so the sums are those of one ray after another.
     // Steps past a ray's length have a zero step entry (seg 0, dp 0, bin 0): w stays, so sn = surv
     // and they deposit exactly 0 into bin 0 of their shell, whatever density they read; an
-    // inactive lane is dead from the start. Only when counting visits do such steps read +inf
+    // inactive lane is dead from the start. Only when counting visits do such steps read kHalo
     // (which counts no visit).
     for (int r = 0; r < kNearRays; r += 2) {
         int len[2];
@@ -630,7 +635,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
                 ns[h][s] = NearStep{__int_as_float(e[0]), e[1], e[2], e[3]};
             }
         }
-        // The whole ray stays inside the 5x5x5 block (the +inf margin around the padded density
+        // The whole ray stays inside the 5x5x5 block (the kHalo margin around the padded density
         // keeps every address in bounds). Densities of steps 0-3 are loaded up front, those of
         // steps 4-6 only for ray pairs that reach them, while step 3 runs.
         auto load_rho = [&](int s0, int s1) {
@@ -638,7 +643,7 @@ __device__ __attribute__((always_inline)) inline void near_rays(const Vol& V, co
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int s = s0; s < s1; ++s)
-                    rv[h][s] = !COUNT ? rho_at(ns[h][s].dp) : (s < len[h] && act) ? rho_at(ns[h][s].dp) : INFINITY;
+                    rv[h][s] = !COUNT ? rho_at(ns[h][s].dp) : (s < len[h] && act) ? rho_at(ns[h][s].dp) : kHalo;
         };
         load_rho(0, 4);
         Lane st[2] = {M.start(act), M.start(act)};
@@ -900,7 +905,7 @@ __global__ __launch_bounds__(256) void k_near_fill(const Vol* __restrict__ vols,
 }
 
 // Near-block memo, classification. A source is uniform when all 125 densities of its 5x5x5
-// block equal its own; a block reaching outside the grid includes the +inf halo or margin and
+// block equal its own; a block reaching outside the grid includes the kHalo halo or margin and
 // never is. Uniform densities get template ids from a 64-slot table (first come, first served:
 // the ids only name templates, the weights do not depend on them), and the representative of
 // each template is its lowest voxel index. cls: -2 inactive, -1 marched, t >= 0 template t.
@@ -1073,12 +1078,12 @@ __global__ __launch_bounds__(256) void k_far_count(const Vol* __restrict__ vols,
 #pragma unroll
             for (int u = 0; u < kCountGroup; ++u) {
                 const float rho = rv[u];
-                const bool inside = rho < INFINITY, tissue = rho >= support_rho, alive = w < cutoff;
-                if (COUNT) visits += (inside && alive) ? 1u : 0u;
-                lg = (inside && tissue && alive) ? u : lg;
+                const bool tissue = rho >= support_rho, alive = w < cutoff;  // kHalo: not tissue
+                if (COUNT) visits += (rho < INFINITY && alive) ? 1u : 0u;
+                lg = (tissue && alive) ? u : lg;
                 air = tissue ? 0.0f : air + seg[u];
-                // Death in air; a step onto the +inf halo kills too, as fminf(fmaf(inf, seg, w),
-                // cutoff) = cutoff (for seg = 0 as well: fminf returns the non-NaN operand).
+                // Death in air; a step onto the kHalo halo kills too, as fminf(fmaf(NaN, seg, w),
+                // cutoff) = cutoff.
                 w = air > air_cap ? cutoff : fminf(fmaf(rho, seg[u], w), cutoff);
             }
             last = lg >= 0 ? s0 + lg : last;
@@ -2411,7 +2416,7 @@ struct prc_projector {
     Isotope model{};
     Vol hv{};
     Vol* dv = nullptr;
-    float* rho_alloc = nullptr;  // padded density with its +inf margin
+    float* rho_alloc = nullptr;  // padded density with its kHalo margin
     float* rho_p = nullptr;      // first voxel of the padded volume inside rho_alloc
     float* table = nullptr;
     unsigned char* support = nullptr;
@@ -2660,10 +2665,10 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     A->stats.n_voxels = (long long)n;
     A->wbytes = g.precision == PRC_PRECISION_MIXED ? 2 : 4;
 
-    // Density: clamp (prc.cu:603), support mask (prc.cu:605), +inf halo of width one voxel.
+    // Density: clamp (prc.cu:603), support mask (prc.cu:605), kHalo halo of width one voxel.
     const int nxp = g.nx + 2, nyp = g.ny + 2, nzp = g.nz + 2;
     const size_t np = (size_t)nxp * nyp * nzp;
-    // +inf margin on both sides for the march's group loads: every lane of a wave loads the
+    // kHalo margin on both sides for the march's group loads: every lane of a wave loads the
     // group's voxels, and a group runs only while some lane of the wave is alive. That lane's
     // last voxel is inside the grid and a group moves it at most kMarchGroup voxels, at most
     // kMarchGroup padded slices; the other lanes of the wave are at most 63 voxels further along
@@ -2671,7 +2676,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     const size_t margin = (size_t)(kMarchGroup + 1) * nxp * nyp + 2 * kWave;
     if ((np + 2 * margin) * sizeof(float) > (size_t)INT_MAX)
         fail(PRC_ERR_INVALID, "Volume too large for 32-bit density addressing");
-    std::vector<float> rp(np + 2 * margin, INFINITY);
+    std::vector<float> rp(np + 2 * margin, kHalo);
     A->h_support.assign(n, 0);
     long long n_active = 0;
     int box[6] = {g.nx, -1, g.ny, -1, g.nz, -1};
