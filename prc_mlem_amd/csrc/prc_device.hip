@@ -3258,8 +3258,7 @@ struct Batch {
     const Vol* dv = nullptr;
     const int4* items_all = nullptr;
     const int4* items_act = nullptr;
-    const int4* items_fwd = nullptr;
-    int n_all = 0, n_act = 0, n_fwd = 0, n_vols = 1, wbytes = 4;
+    int n_all = 0, n_act = 0, n_vols = 1, wbytes = 4;
     int* queue = nullptr;  // work-queue counters of the projector passes
     PassShape fwd_shape = kShapes[kDefaultShape], adj_shape = kShapes[kDefaultShape];  // chosen by tune_passes
     const int* item_begin = nullptr;  // per-volume ranges of items_all
@@ -3306,7 +3305,11 @@ void batch_iteration(const Batch& B) {
             k_half_copy<<<nb, 256, 0, B.stream>>>(B.dv, B.items_all, B.n_all, B.sc, B.gf, B.gh, 0);
         }
     }
-    run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
+    // The forward covers the active segments only. Voxels outside them are left as they were:
+    // k_ratio uses the forward on support voxels only and rewrites every voxel of est (0 off
+    // support), so nothing reads them (the operators' forward, which must write every voxel, adds
+    // the zero pieces of each row).
+    run_forward(B.dv, B.items_act, B.n_act, B.wbytes, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.queue,
                 B.fwd_shape, B.stream);
     run_far_march(B.dv, B.march, B.wbytes, kMarchFwd, B.gh ? (const void*)B.gh : B.gf, B.gh != nullptr, B.est, B.stream);
     {
@@ -3368,13 +3371,13 @@ void tune_passes(Batch& B) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    TuneRecord r{B.n_vols, B.wbytes, B.n_fwd, B.n_act, {}, {kDefaultShape, kDefaultShape}};
+    TuneRecord r{B.n_vols, B.wbytes, B.n_act, B.n_act, {}, {kDefaultShape, kDefaultShape}};
     try {
         for (int op = 0; op < 2; ++op)
             for (int k = 0; k < kShapeCount; ++k) {
                 auto launch = [&] {
                     if (op == 0)
-                        run_forward(B.dv, B.items_fwd, B.n_fwd, B.wbytes, B.gh ? (const void*)B.gh : B.gf,
+                        run_forward(B.dv, B.items_act, B.n_act, B.wbytes, B.gh ? (const void*)B.gh : B.gf,
                                     B.gh != nullptr, B.est, B.queue, kShapes[k], B.stream);
                     else
                         run_adjoint(B.dv, B.items_act, B.n_act, B.wbytes, B.rh ? (const void*)B.rh : B.est,
@@ -3864,7 +3867,7 @@ static Batch solver_batch(prc_solver* S) {
     Batch B;
     prc_projector* A = S->A;
     B.dv = A->dv, B.items_all = A->items_all, B.items_act = A->items_act, B.n_all = A->n_all, B.n_act = A->n_act;
-    B.items_fwd = A->items_fwd, B.n_fwd = A->n_fwd, B.queue = A->queue;
+    B.queue = A->queue;
     B.fwd_shape = A->fwd_shape, B.adj_shape = A->adj_shape;
     B.n_vols = 1, B.wbytes = A->wbytes, B.item_begin = S->item_begin;
     B.tiles = A->tiles, B.n_tiles = A->n_tiles, B.tile_begin = S->tile_begin;
@@ -3943,7 +3946,7 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             b->A.assign(projectors, projectors + n);
             b->vols.resize(n);
             b->sc0.resize(n);
-            std::vector<int4> all, act, fwd, tiles;
+            std::vector<int4> all, act, tiles;
             std::vector<int> begin(n + 1, 0), tbegin(n + 1, 0);
             const int wbytes = projectors[0] ? projectors[0]->wbytes : 4;
             const bool accel = params[0].accelerate != 0;
@@ -3962,7 +3965,6 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
                 for (const int4& t : A->h_tiles) tiles.push_back(make_int4(v, t.y, t.z, t.w));
                 for (const int4& it : A->h_items_all) all.push_back(make_int4(v, it.y, it.z, it.w));
                 for (const int4& it : A->h_items_act) act.push_back(make_int4(v, it.y, it.z, it.w));
-                for (const int4& it : A->h_items_fwd) fwd.push_back(make_int4(v, it.y, it.z, it.w));
                 b->total += A->n + 2 * (size_t)A->margin;
             }
             begin[n] = (int)all.size();
@@ -3981,8 +3983,6 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
             b->dv = (Vol*)own(dalloc<Vol>(n));
             int4* d_all = (int4*)own(dalloc<int4>(all.size()));
             int4* d_act = (int4*)own(dalloc<int4>(std::max<size_t>(act.size(), 1)));
-            int4* d_fwd = (int4*)own(dalloc<int4>(std::max<size_t>(fwd.size(), 1)));
-            HIPCHK(hipMemcpy(d_fwd, fwd.data(), fwd.size() * sizeof(int4), hipMemcpyHostToDevice));
             int* d_begin = (int*)own(dalloc<int>(n + 1));
             HIPCHK(hipMemcpy(d_all, all.data(), all.size() * sizeof(int4), hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(d_act, act.data(), act.size() * sizeof(int4), hipMemcpyHostToDevice));
@@ -4002,7 +4002,6 @@ int prc_batch_create(int n, prc_projector* const* projectors, const float* const
                 B.march.push_back(MarchJob{A, d, (int)mine.size()});
             }
             B.dv = b->dv, B.items_all = d_all, B.items_act = d_act, B.n_all = (int)all.size(), B.n_act = (int)act.size();
-            B.items_fwd = d_fwd, B.n_fwd = (int)fwd.size();
             B.queue = (int*)own(dalloc<int>(kXcds));
             B.n_vols = n, B.wbytes = wbytes, B.item_begin = d_begin, B.accel = accel, B.any_prior = prior;
             B.g = (float*)own(dalloc<float>(b->total));
