@@ -966,31 +966,56 @@ __global__ __launch_bounds__(256) void k_classify(const Vol* __restrict__ vols, 
 }
 
 // The march list of the memo: marched sources and template representatives, appended per block
-// (one atomic per block; the order only changes which lanes march together). Also counts the
+// (one atomic per block of kMemoChunks x 256 voxels: the blocks' appends to one counter serialise,
+// so fewer and larger blocks; the order only changes which lanes march together). Also counts the
 // uniform sources.
 constexpr int kMemoBlock = 256;
+constexpr int kMemoChunks = 16;
+constexpr int kMemoWaves = kMemoBlock / kWave;
 __global__ __launch_bounds__(kMemoBlock) void k_memo_list(int n, const signed char* __restrict__ cls,
                                                           const int* __restrict__ rep, int2* __restrict__ list,
                                                           int* __restrict__ ctr) {
-    __shared__ int wt[kMemoBlock / kWave], wu[kMemoBlock / kWave], base;
-    const long long vl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int wt[kMemoChunks * kMemoWaves], base;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    const int v = (int)min(vl, (long long)n - 1);
-    const int c = vl < n ? (int)cls[v] : -2;
-    const bool take = c == -1 || (c >= 0 && rep[c] == v);
-    const unsigned long long b = __ballot(take), u = __ballot(c >= 0);
-    if (lane == 0) wt[wv] = __popcll(b), wu[wv] = __popcll(u);
+    const long long v0 = (long long)blockIdx.x * kMemoChunks * kMemoBlock + threadIdx.x;
+    auto taken = [&](int j, int& v, int& c) {
+        const long long vl = v0 + (long long)j * kMemoBlock;
+        v = (int)min(vl, (long long)n - 1);
+        c = vl < n ? (int)cls[v] : -2;
+        return c == -1 || (c >= 0 && rep[c] == v);
+    };
+    // Pass 1: the taken and uniform voxels of each chunk, per wave.
+    int us = 0;
+    for (int j = 0; j < kMemoChunks; ++j) {
+        int v, c;
+        const unsigned long long b = __ballot(taken(j, v, c));
+        us += __popcll(__ballot(c >= 0));
+        if (lane == 0) wt[j * kMemoWaves + wv] = __popcll(b);
+    }
+    __shared__ int wu[kMemoWaves];
+    if (lane == 0) wu[wv] = us;
     __syncthreads();
     if (threadIdx.x == 0) {
-        int t = 0, us = 0;
-        for (int k = 0; k < kMemoBlock / kWave; ++k) t += wt[k], us += wu[k];
+        int t = 0, u = 0;
+        for (int k = 0; k < kMemoChunks * kMemoWaves; ++k) t += wt[k];
+        for (int k = 0; k < kMemoWaves; ++k) u += wu[k];
         base = t ? atomicAdd(ctr, t) : 0;
-        if (us) atomicAdd(ctr + 1, us);
+        if (u) atomicAdd(ctr + 1, u);
     }
     __syncthreads();
+    // Pass 2: each chunk's taken voxels in order, after those of the chunks and waves before it.
     int off = base;
     for (int k = 0; k < wv; ++k) off += wt[k];
-    if (take) list[off + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
+    for (int j = 0; j < kMemoChunks; ++j) {
+        int v, c;
+        const bool take = taken(j, v, c);
+        const unsigned long long b = __ballot(take);
+        if (take) list[off + __popcll(b & ((1ull << lane) - 1))] = make_int2(v, c);
+        if (j + 1 < kMemoChunks) {
+            for (int k = wv; k < kMemoWaves; ++k) off += wt[j * kMemoWaves + k];
+            for (int k = 0; k < wv; ++k) off += wt[(j + 1) * kMemoWaves + k];
+        }
+    }
 }
 
 // Far rays, count pass: the number of stored steps of every (segment, ray), i.e. one past the last
@@ -2849,7 +2874,7 @@ void normalise_begin(prc_projector* A, hipStream_t st) {
         HIPCHK(hipMemsetAsync(keys + 2 * kMaxTemplates, 0, 2 * sizeof(int), st));
         const unsigned nb = blocks_for((long long)A->n, 256);
         k_classify<<<nb, 256, 0, st>>>(A->dv, A->cls, (unsigned*)keys, keys + kMaxTemplates);
-        k_memo_list<<<blocks_for((long long)A->n, kMemoBlock), kMemoBlock, 0, st>>>(
+        k_memo_list<<<blocks_for((long long)A->n, kMemoChunks * kMemoBlock), kMemoBlock, 0, st>>>(
             (int)A->n, A->cls, keys + kMaxTemplates, A->near_list, keys + 2 * kMaxTemplates);
         HIPCHK(hipGetLastError());
     }
