@@ -129,7 +129,7 @@ float range_cutoff_host(const Isotope& m, float fraction) {
 struct RowInfo {        // one per (segment, ray); 32 bytes, one scalar load
     long long off;      // element offset of this segment's planes for this ray in the weight buffer
     int cnt;            // stored steps (planes)
-    int kt;             // forward bound: last step (exclusive) at which this segment receives weight
+    int pad2;
     int xa, len;        // the segment's x range [xa, xa + len) = plane width (len 0: empty segment)
     int pad0, pad1;
 };
@@ -1531,11 +1531,12 @@ __global__ void k_row_offsets(Vol V, const long long* __restrict__ unit_base, co
 }
 
 // kglob[p] = kfirst[p] + max over segments of cnt(segment, p): the forward's search bound per ray.
-__global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
+__global__ __launch_bounds__(256) void k_kglob(Vol V, const int* __restrict__ cnt_t, int* __restrict__ kglob) {
     __shared__ int sm[256];
-    const int p = blockIdx.x, R1 = V.n_rays + 1, units = V.ny * V.nz * V.nseg;
+    const int p = blockIdx.x, units = V.ny * V.nz * V.nseg;
+    const int* cnt_p = cnt_t + (size_t)p * units;  // the ray's counts, one per segment (coalesced)
     int m = 0;
-    for (int u = threadIdx.x; u < units; u += blockDim.x) m = max(m, V.rinfo[(size_t)u * R1 + p].cnt);
+    for (int u = threadIdx.x; u < units; u += blockDim.x) m = max(m, cnt_p[u]);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
@@ -1548,8 +1549,10 @@ __global__ __launch_bounds__(256) void k_kglob(Vol V, int* __restrict__ kglob) {
 // Forward bound per (segment, ray): one past the last step at which some source segment that
 // stores that step reaches this segment's x range (an upper bound of where deposits land).
 // Threads take (ray, segment) with the segment fastest, so neighbouring threads share the ray's
-// steps and read neighbouring entries of the ray-major counts.
-__global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int* __restrict__ cnt_t) {
+// steps, read neighbouring entries of the ray-major counts and write neighbouring entries of the
+// ray-major bounds kt_t (which k_target_totals turns into the target-major runs).
+__global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int* __restrict__ cnt_t,
+                                 int* __restrict__ kt_t) {
     const int R1 = V.n_rays + 1, nseg = V.nseg, units = V.ny * V.nz * nseg;
     const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (long long)units * R1) return;
@@ -1594,25 +1597,23 @@ __global__ void k_forward_bounds(Vol V, const int* __restrict__ kglob, const int
                 if (kt == 0 && (k - u - kf < c0[u] || k - u - kf < c1[u])) kt = k - u + 1;
         }
     }
-    V.rinfo[(size_t)unit * R1 + p].kt = kt;
+    kt_t[id] = kt;  // ray-major, as the threads run: [p][unit]
 }
 
 // Target-major layout: segment t holds, for ray p, the steps kfirst(p) .. kt(t, p) - 1 at which some
 // source deposits into it (the near block: all 125 bins); planes over the segment's x range, runs
 // without gaps as in the source-major copy.
 __global__ __launch_bounds__(256) void k_target_totals(Vol V, long long* unit_tot, long long* unit_planes,
-                                                        PlaneRun* runs) {
-    const int units = V.ny * V.nz * V.nseg, R1 = V.n_rays + 1;
+                                                        PlaneRun* runs, const int* __restrict__ kt_t) {
+    const int units = V.ny * V.nz * V.nseg;
     const int unit = blockIdx.x * (blockDim.x / kWave) + wave_in_block();
     if (unit >= units) return;
     const int row = unit / V.nseg, seg = unit % V.nseg;
     const int w = segment_width(gload(V.row_len + row), seg), x0 = gload(V.row_xa + row) + kWave * seg;
-    for (int p = threadIdx.x & (kWave - 1); p < R1; p += kWave)
-        V.tinfo[(size_t)unit * R1 + p].kt = V.rinfo[(size_t)unit * R1 + p].kt;
     long long entries;
     int planes;
     layout_runs(V, V.tinfo, unit, w, x0, [&](int p) {
-        return p == 0 ? kNQ : max(0, V.rinfo[(size_t)unit * R1 + p].kt - gload(V.kfirst + p));
+        return p == 0 ? kNQ : max(0, kt_t[(size_t)p * units + unit] - gload(V.kfirst + p));
     }, entries, planes);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         unit_tot[unit] = align_up(entries, kWave);
@@ -2471,6 +2472,7 @@ struct prc_projector {
     long long* unit_planes = nullptr;  // per segment: list length (aligned)
     long long* tile_sums = nullptr;    // layout scan: sums of segment tiles (two arrays)
     int* cnt_t = nullptr;              // [n_rays + 1][segments] source-major counts, ray-major
+    int* kt_t = nullptr;               // [n_rays + 1][segments] forward bounds kt(t, p), ray-major
     TargetRun* tmaj = nullptr;         // [n_rays + 1][segments] target-major runs, ray-major
     long long* list_base = nullptr;
     long long* total = nullptr;  // entries (source, target), list lengths (source, target)
@@ -2799,6 +2801,7 @@ void setup_projector(prc_projector* A, const float* host_rho, const float* host_
     A->unit_planes = dalloc<long long>(units);
     A->tile_sums = dalloc<long long>(2 * blocks_for((long long)units, kScanTile));
     A->cnt_t = dalloc<int>(units * R1);
+    A->kt_t = dalloc<int>(units * R1);
     A->tmaj = dalloc<TargetRun>(units * R1);
     A->list_base = dalloc<long long>(units);
     A->sruns = dalloc<PlaneRun>(units);
@@ -2904,9 +2907,9 @@ void layout_and_sizes(prc_projector* A, hipStream_t st) {
         k_row_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
         scan_units(A, A->total, A->total + 2, st);
         k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
-        k_kglob<<<R1, 256, 0, st>>>(V, A->kglob);
-        k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t);
-        k_target_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns);
+        k_kglob<<<R1, 256, 0, st>>>(V, A->cnt_t, A->kglob);
+        k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t, A->kt_t);
+        k_target_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns, A->kt_t);
         scan_units(A, A->total + 1, A->total + 3, st);
         k_target_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->truns, A->tmaj);
         HIPCHK(hipGetLastError());
@@ -3159,7 +3162,7 @@ void release(prc_projector* A) {
     dfree(A->row_len), dfree(A->rinfo), dfree(A->fstep), dfree(A->fsoa), dfree(A->pstep), dfree(A->kfirst), dfree(A->nstep);
     dfree(A->nlen), dfree(A->items_act), dfree(A->items_all), dfree(A->items_fwd), dfree(A->tiles), dfree(A->queue), dfree(A->counters), dfree(A->scratch_in);
     dfree(A->kglob), dfree(A->row_tot), dfree(A->row_base), dfree(A->total), dfree(A->tinfo), dfree(A->plin);
-    dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->tmaj), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
+    dfree(A->unit_planes), dfree(A->tile_sums), dfree(A->cnt_t), dfree(A->kt_t), dfree(A->tmaj), dfree(A->list_base), dfree(A->sruns), dfree(A->truns), dfree(A->slist), dfree(A->tlist);
     dfree(A->scratch_adj), dfree(A->cls), dfree(A->near_list), dfree(A->tmpl), dfree(A->tmpl_sum), dfree(A->memo_meta);
     if (A->wraw) (void)hipFree(A->wraw);
     if (A->traw) (void)hipFree(A->traw);
