@@ -1530,20 +1530,24 @@ __global__ void k_row_offsets(Vol V, const long long* __restrict__ unit_base, co
     if (id % R1 == 0) runs[unit].off = unit_base[unit], runs[unit].dbase = (int)list_base[unit];
 }
 
-// kglob[p] = kfirst[p] + max over segments of cnt(segment, p): the forward's search bound per ray.
+// kglob[p]: one past the last stored step of ray p over all segments (0 if it stores none), as
+// kfirst(p) + the largest count. Blocks take (ray, a chunk of kKglobChunks of the segments) and
+// fold their maximum in with one atomicMax (kglob starts at zero).
+constexpr int kKglobChunks = 8;
 __global__ __launch_bounds__(256) void k_kglob(Vol V, const int* __restrict__ cnt_t, int* __restrict__ kglob) {
     __shared__ int sm[256];
-    const int p = blockIdx.x, units = V.ny * V.nz * V.nseg;
+    const int p = blockIdx.x / kKglobChunks, chunk = blockIdx.x % kKglobChunks, units = V.ny * V.nz * V.nseg;
     const int* cnt_p = cnt_t + (size_t)p * units;  // the ray's counts, one per segment (coalesced)
+    const int u0 = (int)((long long)units * chunk / kKglobChunks), u1 = (int)((long long)units * (chunk + 1) / kKglobChunks);
     int m = 0;
-    for (int u = threadIdx.x; u < units; u += blockDim.x) m = max(m, cnt_p[u]);
+    for (int u = u0 + threadIdx.x; u < u1; u += blockDim.x) m = max(m, cnt_p[u]);
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) sm[threadIdx.x] = max(sm[threadIdx.x], sm[threadIdx.x + o]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) kglob[p] = sm[0] > 0 ? gload(V.kfirst + p) + sm[0] : 0;
+    if (threadIdx.x == 0 && sm[0] > 0) atomicMax(kglob + p, gload(V.kfirst + p) + sm[0]);
 }
 
 // Forward bound per (segment, ray): one past the last step at which some source segment that
@@ -2907,7 +2911,8 @@ void layout_and_sizes(prc_projector* A, hipStream_t st) {
         k_row_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->sruns, A->cnt_t);
         scan_units(A, A->total, A->total + 2, st);
         k_row_offsets<<<nr, 256, 0, st>>>(V, A->row_base, A->list_base, A->sruns);
-        k_kglob<<<R1, 256, 0, st>>>(V, A->cnt_t, A->kglob);
+        HIPCHK(hipMemsetAsync(A->kglob, 0, R1 * sizeof(int), st));
+        k_kglob<<<R1 * kKglobChunks, 256, 0, st>>>(V, A->cnt_t, A->kglob);
         k_forward_bounds<<<nr, 256, 0, st>>>(V, A->kglob, A->cnt_t, A->kt_t);
         k_target_totals<<<nw, 256, 0, st>>>(V, A->row_tot, A->unit_planes, A->truns, A->kt_t);
         scan_units(A, A->total + 1, A->total + 3, st);
